@@ -1,0 +1,160 @@
+/* HIP backend for tile low-rank (TLR) compression on one MI355X (gfx950).
+ *
+ * Two drivers have the reference approximation signature
+ * (include/starsh.h:563-564 in ecrc/stars-h 0.3.1) and are selected by
+ * starsh_init() through STARSH_BACKEND=HIP / HIP_MIXED:
+ *
+ *   starsh_blrm__drsdd_hip        plain form: FP64 randomized SVD of every
+ *                                 far-field tile with sketch width
+ *                                 k = min(maxrank+oversample, m, n), the same
+ *                                 dlarnv test matrix and the same rank rule as
+ *                                 src/backends/openmp/blrm/drsdd.c:16-307 and
+ *                                 src/backends/sequential/dense/drsdd.c:16-93.
+ *   starsh_blrm__drsdd_hip_mixed  batched mixed-precision form: sketch in
+ *                                 exact int8-sliced MFMA arithmetic, rank
+ *                                 buckets for the sketch width, FP64
+ *                                 re-orthogonalisation, core SVD and checks.
+ *
+ * starsh_blrm__dfe_hip replaces src/backends/openmp/blrm/dfe.c:16-148.
+ *
+ * Tile contents are generated on the GPU from the problem's point data.  The
+ * driver recognises the kernel by the identity of F->problem->kernel (one of
+ * this library's STARSH_kernel functions); any other kernel pointer is
+ * rejected with STARSH_WRONG_PARAMETER.
+ *
+ * Memory contract (as the reference): U/V/D are written to malloc'd host
+ * buffers owned by the returned STARSH_blrm and released by
+ * starsh_blrm_free().  *format is rewritten in place when false far-field
+ * tiles exist. */
+#ifndef STARSH_HIP_H
+#define STARSH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+struct starsh_blrm;
+struct starsh_blrf;
+
+/* Plain FP64 driver; replaces starsh_blrm__drsdd_omp
+ * (src/backends/openmp/blrm/drsdd.c:16-307). */
+int starsh_blrm__drsdd_hip(struct starsh_blrm **matrix,
+        struct starsh_blrf *format, int maxrank, double tol, int onfly);
+
+/* Batched mixed-precision driver; same signature and output contract. */
+int starsh_blrm__drsdd_hip_mixed(struct starsh_blrm **matrix,
+        struct starsh_blrf *format, int maxrank, double tol, int onfly);
+
+/* Global relative Frobenius error on the GPU; replaces starsh_blrm__dfe_omp
+ * (src/backends/openmp/blrm/dfe.c:16-148).  Returns -1 on failure. */
+double starsh_blrm__dfe_hip(struct starsh_blrm *matrix);
+
+/* Per-tile variant of starsh_blrm__dfe_hip (no reference counterpart): fills
+ * far_err[bi] = |A_bi - U_bi V_bi^T|_F / |A_bi|_F (0 when |A_bi|_F == 0) and
+ * far_norm[bi] = |A_bi|_F for every far-field tile, and returns the global
+ * error.  Either array may be NULL. */
+double starsh_blrm__dfe_hip_tiles(struct starsh_blrm *matrix,
+        double *far_err, double *far_norm);
+
+/* Run statistics of the last HIP compression on this thread (no reference
+ * counterpart; the reference keeps only internal timers,
+ * src/backends/openmp/blrm/drsdd.c:128-140). */
+typedef struct starsh_hip_stats
+{
+    double device_seconds;   /* first kernel to last kernel, HIP events */
+    double wall_seconds;     /* whole API call incl. copy-back */
+    double copy_seconds;     /* device-to-host copy of U, V and D */
+    double dominant_kernel_seconds; /* summed duration of the sketch kernel */
+    long long dominant_kernel_launches;
+    long long ntiles;        /* far-field tiles compressed */
+    long long nfalse_far;    /* tiles that turned out to be dense */
+    int rank_min, rank_max;
+    double rank_avg;
+    double flops_algorithmic;   /* work of the reference algorithm */
+    double sketch_bytes_or_flops; /* algorithmic flops of the sketch kernel */
+} starsh_hip_stats;
+
+int starsh_hip_get_stats(starsh_hip_stats *stats);
+
+/* Set to 1 to keep the last compression's factors resident on the device
+ * after the copy-back, so that a following starsh_blrm__dfe_hip skips the
+ * upload (no reference counterpart). */
+int starsh_hip_set_option(const char *name, double value);
+
+/* ---- Thin C ABI over the HIP kernels.  Plain pointers and sizes; device
+ * pointers are marked d_.  Every function returns STARSH_SUCCESS or a
+ * STARSH_ERRNO code and never aborts. */
+
+/* Kernel families the device generators implement. */
+enum STARSH_HIP_KERNEL_FAMILY
+{
+    STARSH_HIP_EXP = 1,       /* sigma*exp(-r/beta); r==0 -> sigma+noise */
+    STARSH_HIP_SQREXP = 2,    /* sigma*exp(-r^2/(2 beta^2)) */
+    STARSH_HIP_MATERN = 3,    /* x = sqrt(2 nu) r/beta, nu in {.5,1.5,2.5} */
+    STARSH_HIP_MATERN2 = 4,   /* x = r/beta */
+    STARSH_HIP_COULOMB = 5,   /* 1/r; r==0 -> 0 */
+    STARSH_HIP_SIN = 6,       /* sin(k r)/r; r==0 -> diag */
+    STARSH_HIP_COS = 7        /* cos(k r)/r; r==0 -> diag */
+};
+
+typedef struct starsh_hip_kernel_desc
+{
+    int family;     /* enum STARSH_HIP_KERNEL_FAMILY */
+    int ndim;       /* 1..4 */
+    double beta;
+    double nu;
+    double noise;
+    double sigma;
+    double wave_k;
+    double diag;
+} starsh_hip_kernel_desc;
+
+/* Selects the device and creates the library stream.  Called lazily by the
+ * drivers; exposed for callers that want to pick a device. */
+int starsh_hip_init(int device);
+
+/* Fills d_omega (ncols x k, column-major, leading dimension ncols) with the
+ * first ncols*k values of LAPACK dlarnv(idist=3, iseed={0,0,0,1}) — the
+ * test matrix of src/backends/sequential/dense/drsdd.c:57-59. */
+int starsh_hip_omega(double *d_omega, int ncols, int k);
+
+/* Generates ntiles dense tiles.  Tile t covers rows [r0[t], r0[t]+m[t]) of
+ * d_rowpts and columns [c0[t], c0[t]+n[t]) of d_colpts (points already in
+ * cluster order, SoA with strides nrowpts / ncolpts) and is written
+ * column-major with leading dimension m[t] at d_out + off[t].  The host
+ * arrays r0, c0, m, n, off have ntiles entries. */
+int starsh_hip_generate_tiles(const starsh_hip_kernel_desc *kd,
+        const double *d_rowpts, int64_t nrowpts,
+        const double *d_colpts, int64_t ncolpts, int ntiles,
+        const int64_t *r0, const int64_t *c0, const int *m, const int *n,
+        const int64_t *off, double *d_out);
+
+/* Raw device memory helpers used by the drivers and by tests. */
+int starsh_hip_malloc(void **d_ptr, size_t bytes);
+int starsh_hip_free(void *d_ptr);
+int starsh_hip_memcpy_h2d(void *d_dst, const void *src, size_t bytes);
+int starsh_hip_memcpy_d2h(void *dst, const void *d_src, size_t bytes);
+int starsh_hip_synchronize(void);
+
+/* Dense FP64 randomized-SVD of ntiles explicit tiles already in device
+ * memory (tile t: m x n, column-major, leading dimension m, at d_tiles +
+ * off[t]).  Same algorithm and rank rule as starsh_dense_dlrrsdd
+ * (src/backends/sequential/dense/drsdd.c:16-93) with sketch width
+ * k = min(maxrank+oversample, m, n); rank[t] = -1 marks a tile whose rank
+ * exceeds maxrank.  U (m x rank) and V (n x rank, V = S*Vs^T rows) are
+ * written to host arrays U + uoff[t], V + voff[t]; singular values (k per
+ * tile) to S + soff[t] when S is not NULL.  This is the per-tile
+ * entry point the parity tests use for arbitrary matrices. */
+int starsh_hip_dense_rsdd(int ntiles, const int *m, const int *n,
+        const double *d_tiles, const int64_t *off, int maxrank,
+        int oversample, double tol, int *rank, double *U,
+        const int64_t *uoff, double *V, const int64_t *voff, double *S,
+        const int64_t *soff);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

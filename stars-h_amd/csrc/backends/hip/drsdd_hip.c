@@ -1,0 +1,540 @@
+/* Plain FP64 HIP driver of tile low-rank compression.
+ *
+ * Follows src/backends/openmp/blrm/drsdd.c:16-307 of ecrc/stars-h 0.3.1:
+ *   1. allocate U/V host buffers maxrank columns wide per far-field tile
+ *      (:53-90);
+ *   2. compress every far-field tile (:94-145) -- here all tiles of a chunk
+ *      go through the batched HIP pipeline at once (rsdd_batch), tile
+ *      contents generated on the GPU from the points;
+ *   3. collect false far-field tiles (rank -1) in ascending order, rebuild
+ *      the format and swap it into *format (:146-212);
+ *   4. generate near-field tiles when onfly == 0 (:214-256), on the GPU;
+ *   5. compact the far-field arrays (:259-282) and build the STARSH_blrm
+ *      (:305-306).
+ * The mixed-precision driver shares steps 1 and 3-5 (hip_driver_finish). */
+#include <time.h>
+#include "common.h"
+#include "kernels_host.h"
+#include "rsdd.h"
+#include "hip_driver.h"
+
+static __thread starsh_hip_stats last_stats;
+static size_t workspace_budget = (size_t)96 << 30;
+
+int starsh_hip_get_stats(starsh_hip_stats *stats)
+{
+    if(stats == NULL)
+        return STARSH_WRONG_PARAMETER;
+    *stats = last_stats;
+    return STARSH_SUCCESS;
+}
+
+void hip_driver_set_stats(const starsh_hip_stats *stats)
+{
+    last_stats = *stats;
+}
+
+int starsh_hip_set_option(const char *name, double value)
+{
+    if(name == NULL)
+        return STARSH_WRONG_PARAMETER;
+    if(!strcmp(name, "workspace_gb"))
+    {
+        if(value <= 0)
+            return STARSH_WRONG_PARAMETER;
+        workspace_budget = (size_t)(value*(double)(1 << 30));
+        return STARSH_SUCCESS;
+    }
+    return STARSH_WRONG_PARAMETER;
+}
+
+size_t hip_driver_workspace_budget(void)
+{
+    return workspace_budget;
+}
+
+double hip_driver_now(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec+1e-9*ts.tv_nsec;
+}
+
+/* Copies the points of a cluster into cluster (pivot) order, SoA, and
+ * uploads them. */
+static int upload_points(const double *x, STARSH_int count, int ndim,
+        const STARSH_cluster *C, double **d_x)
+{
+    STARSH_int n = C->ndata;
+    double *h = malloc(sizeof(double)*(size_t)n*ndim + 1);
+    if(h == NULL)
+        return STARSH_MALLOC_ERROR;
+    for(int d = 0; d < ndim; d++)
+        for(STARSH_int i = 0; i < n; i++)
+        {
+            STARSH_int p = C->pivot[i];
+            if(p < 0 || p >= count)
+            {
+                free(h);
+                STARSH_ERROR("pivot entry out of range");
+                return STARSH_WRONG_PARAMETER;
+            }
+            h[(size_t)d*n+i] = x[(size_t)d*count+p];
+        }
+    int info = starsh_hip_malloc((void **)d_x, sizeof(double)*(size_t)n*ndim);
+    if(info == STARSH_SUCCESS)
+        info = starsh_hip_memcpy_h2d(*d_x, h, sizeof(double)*(size_t)n*ndim);
+    free(h);
+    return info;
+}
+
+int hip_driver_prepare(STARSH_blrf *F, int maxrank, double tol, int onfly,
+        hip_problem *hp)
+{
+    memset(hp, 0, sizeof(*hp));
+    if(F == NULL || F->problem == NULL)
+    {
+        STARSH_ERROR("Invalid value of `format`");
+        return STARSH_WRONG_PARAMETER;
+    }
+    if(maxrank <= 0 || !(tol >= 0) || (onfly != 0 && onfly != 1))
+    {
+        STARSH_ERROR("Invalid value of `maxrank`, `tol` or `onfly`");
+        return STARSH_WRONG_PARAMETER;
+    }
+    STARSH_problem *P = F->problem;
+    if(P->ndim != 2 || P->dtype != 'd')
+    {
+        STARSH_ERROR("Only scalar double precision problems are supported");
+        return STARSH_WRONG_PARAMETER;
+    }
+    const double *xr, *xc;
+    STARSH_int cr, cc;
+    starsh_hip_kernel_desc kd2;
+    if(starsh_kernel_describe(P->kernel, P->row_data, &hp->kd, &xr, &cr) != 0
+            || starsh_kernel_describe(P->kernel, P->col_data, &kd2, &xc,
+                &cc) != 0)
+    {
+        STARSH_ERROR("The problem kernel has no HIP generator (only this "
+                "library's spatial exp/sqrexp/matern/matern2, "
+                "electrostatics and electrodynamics kernels with ndim <= 4 "
+                "are supported)");
+        return STARSH_WRONG_PARAMETER;
+    }
+    int info = starsh_hip_init(0);
+    if(info != STARSH_SUCCESS)
+        return info;
+    hp->oversample = starsh_params.oversample > 0 ? starsh_params.oversample
+        : 10;
+    STARSH_cluster *RC = F->row_cluster, *CC = F->col_cluster;
+    info = upload_points(xr, cr, hp->kd.ndim, RC, &hp->d_xr);
+    if(info != STARSH_SUCCESS)
+        return info;
+    hp->nr = RC->ndata;
+    if(CC == RC && P->col_data == P->row_data)
+    {
+        hp->d_xc = hp->d_xr;
+        hp->nc = hp->nr;
+    }
+    else
+    {
+        info = upload_points(xc, cc, hp->kd.ndim, CC, &hp->d_xc);
+        if(info != STARSH_SUCCESS)
+        {
+            hip_driver_release(hp);
+            return info;
+        }
+        hp->nc = CC->ndata;
+    }
+    return STARSH_SUCCESS;
+}
+
+void hip_driver_release(hip_problem *hp)
+{
+    if(hp->d_xc != NULL && hp->d_xc != hp->d_xr)
+        starsh_hip_free(hp->d_xc);
+    if(hp->d_xr != NULL)
+        starsh_hip_free(hp->d_xr);
+    hp->d_xr = hp->d_xc = NULL;
+}
+
+/* Steps 1: host buffers and wrappers (drsdd.c:53-90). */
+int hip_driver_alloc_factors(STARSH_blrf *F, int maxrank, hip_factors *hf)
+{
+    memset(hf, 0, sizeof(*hf));
+    STARSH_int nfar = F->nblocks_far;
+    STARSH_cluster *RC = F->row_cluster, *CC = F->col_cluster;
+    if(nfar == 0)
+        return STARSH_SUCCESS;
+    STARSH_MALLOC(hf->far_U, nfar);
+    STARSH_MALLOC(hf->far_V, nfar);
+    STARSH_MALLOC(hf->far_rank, nfar);
+    STARSH_MALLOC(hf->uoff, nfar);
+    STARSH_MALLOC(hf->voff, nfar);
+    size_t su = 0, sv = 0;
+    for(STARSH_int bi = 0; bi < nfar; bi++)
+    {
+        hf->uoff[bi] = su;
+        hf->voff[bi] = sv;
+        su += (size_t)RC->size[F->block_far[2*bi]]*maxrank;
+        sv += (size_t)CC->size[F->block_far[2*bi+1]]*maxrank;
+    }
+    STARSH_MALLOC(hf->alloc_U, su);
+    STARSH_MALLOC(hf->alloc_V, sv);
+    for(STARSH_int bi = 0; bi < nfar; bi++)
+    {
+        int shape_U[2] = {(int)RC->size[F->block_far[2*bi]], maxrank};
+        int shape_V[2] = {(int)CC->size[F->block_far[2*bi+1]], maxrank};
+        int info = array_from_buffer(hf->far_U+bi, 2, shape_U, 'd', 'F',
+                hf->alloc_U+hf->uoff[bi]);
+        if(info == STARSH_SUCCESS)
+            info = array_from_buffer(hf->far_V+bi, 2, shape_V, 'd', 'F',
+                    hf->alloc_V+hf->voff[bi]);
+        if(info != STARSH_SUCCESS)
+            return info;
+    }
+    return STARSH_SUCCESS;
+}
+
+void hip_driver_free_factors(STARSH_blrf *F, hip_factors *hf)
+{
+    if(hf->far_U)
+        for(STARSH_int bi = 0; bi < F->nblocks_far; bi++)
+            if(hf->far_U[bi])
+            {
+                hf->far_U[bi]->data = NULL;
+                array_free(hf->far_U[bi]);
+            }
+    if(hf->far_V)
+        for(STARSH_int bi = 0; bi < F->nblocks_far; bi++)
+            if(hf->far_V[bi])
+            {
+                hf->far_V[bi]->data = NULL;
+                array_free(hf->far_V[bi]);
+            }
+    free(hf->far_U);
+    free(hf->far_V);
+    free(hf->far_rank);
+    free(hf->alloc_U);
+    free(hf->alloc_V);
+    free(hf->uoff);
+    free(hf->voff);
+    memset(hf, 0, sizeof(*hf));
+}
+
+/* Steps 3-5 (drsdd.c:146-306). */
+int hip_driver_finish(STARSH_blrm **matrix, STARSH_blrf *format,
+        const hip_problem *hp, hip_factors *hf, int onfly,
+        starsh_hip_stats *st)
+{
+    STARSH_blrf *F = format;
+    STARSH_problem *P = F->problem;
+    STARSH_cluster *RC = F->row_cluster, *CC = F->col_cluster;
+    STARSH_int nblocks_far = F->nblocks_far, nblocks_near = F->nblocks_near;
+    STARSH_int new_far = nblocks_far, new_near = nblocks_near;
+    STARSH_int *block_far = F->block_far, *block_near = F->block_near;
+    int *far_rank = hf->far_rank;
+    Array **far_U = hf->far_U, **far_V = hf->far_V, **near_D = NULL;
+    double *alloc_U = hf->alloc_U, *alloc_V = hf->alloc_V, *alloc_D = NULL;
+    STARSH_int nfalse = 0, bi, bj;
+    for(bi = 0; bi < nblocks_far; bi++)
+        if(far_rank[bi] == -1)
+            nfalse++;
+    st->nfalse_far = nfalse;
+    if(nfalse > 0)
+    {
+        STARSH_int *false_far;
+        STARSH_MALLOC(false_far, nfalse);
+        for(bi = 0, bj = 0; bi < nblocks_far; bi++)
+            if(far_rank[bi] == -1)
+                false_far[bj++] = bi;
+        new_near = nblocks_near+nfalse;
+        STARSH_MALLOC(block_near, 2*new_near);
+        for(bi = 0; bi < 2*nblocks_near; bi++)
+            block_near[bi] = F->block_near[bi];
+        for(bi = 0; bi < nfalse; bi++)
+        {
+            block_near[2*(bi+nblocks_near)] = F->block_far[2*false_far[bi]];
+            block_near[2*(bi+nblocks_near)+1] =
+                F->block_far[2*false_far[bi]+1];
+        }
+        new_far = nblocks_far-nfalse;
+        block_far = NULL;
+        if(new_far > 0)
+        {
+            STARSH_MALLOC(block_far, 2*new_far);
+            for(bi = 0, bj = 0; bi < nblocks_far; bi++)
+            {
+                if(bj < nfalse && false_far[bj] == bi)
+                    bj++;
+                else
+                {
+                    block_far[2*(bi-bj)] = F->block_far[2*bi];
+                    block_far[2*(bi-bj)+1] = F->block_far[2*bi+1];
+                }
+            }
+        }
+        STARSH_blrf *F2;
+        int info = starsh_blrf_new_from_coo(&F2, P, F->symm, RC, CC, new_far,
+                block_far, new_near, block_near, F->type);
+        if(info != STARSH_SUCCESS)
+        {
+            free(false_far);
+            return info;
+        }
+        STARSH_blrf tmp = *F;
+        *F = *F2;
+        *F2 = tmp;
+        STARSH_WARNING("`F` was modified due to false far-field blocks");
+        starsh_blrf_free(F2);
+        /* Compaction of the far-field arrays (drsdd.c:259-282); the old
+         * wrappers of false far-field tiles are released. */
+        for(bi = 0, bj = 0; bi < nblocks_far; bi++)
+        {
+            if(far_rank[bi] == -1)
+            {
+                far_U[bi]->data = NULL;
+                array_free(far_U[bi]);
+                far_V[bi]->data = NULL;
+                array_free(far_V[bi]);
+                bj++;
+                continue;
+            }
+            int shape_U[2] = {far_U[bi]->shape[0], far_rank[bi]};
+            int shape_V[2] = {far_V[bi]->shape[0], far_rank[bi]};
+            void *du = far_U[bi]->data, *dv = far_V[bi]->data;
+            far_U[bi]->data = NULL;
+            array_free(far_U[bi]);
+            far_V[bi]->data = NULL;
+            array_free(far_V[bi]);
+            array_from_buffer(far_U+bi-bj, 2, shape_U, 'd', 'F', du);
+            array_from_buffer(far_V+bi-bj, 2, shape_V, 'd', 'F', dv);
+            far_rank[bi-bj] = far_rank[bi];
+        }
+        free(false_far);
+        if(new_far > 0)
+        {
+            STARSH_REALLOC(far_rank, new_far);
+            STARSH_REALLOC(far_U, new_far);
+            STARSH_REALLOC(far_V, new_far);
+        }
+        else
+        {
+            free(far_rank);
+            free(far_U);
+            free(far_V);
+            free(alloc_U);
+            free(alloc_V);
+            far_rank = NULL;
+            far_U = far_V = NULL;
+            alloc_U = alloc_V = NULL;
+        }
+    }
+    /* Near-field tiles (drsdd.c:214-256), generated on the GPU. */
+    if(onfly == 0 && new_near > 0)
+    {
+        STARSH_MALLOC(near_D, new_near);
+        int64_t *r0 = malloc(sizeof(int64_t)*new_near);
+        int64_t *c0 = malloc(sizeof(int64_t)*new_near);
+        int64_t *off = malloc(sizeof(int64_t)*new_near);
+        int *mm = malloc(sizeof(int)*new_near), *nn = malloc(sizeof(int)*
+                new_near);
+        if(!r0 || !c0 || !off || !mm || !nn)
+        {
+            free(r0); free(c0); free(off); free(mm); free(nn);
+            return STARSH_MALLOC_ERROR;
+        }
+        size_t size_D = 0;
+        for(bi = 0; bi < new_near; bi++)
+        {
+            STARSH_int i = block_near[2*bi], j = block_near[2*bi+1];
+            r0[bi] = RC->start[i];
+            c0[bi] = CC->start[j];
+            mm[bi] = (int)RC->size[i];
+            nn[bi] = (int)CC->size[j];
+            off[bi] = size_D;
+            size_D += (size_t)mm[bi]*nn[bi];
+        }
+        int info = STARSH_SUCCESS;
+        alloc_D = malloc(sizeof(double)*size_D + 1);
+        double *d_D = NULL;
+        if(alloc_D == NULL)
+            info = STARSH_MALLOC_ERROR;
+        if(info == STARSH_SUCCESS)
+            info = starsh_hip_malloc((void **)&d_D, sizeof(double)*size_D);
+        if(info == STARSH_SUCCESS)
+            info = starsh_hip_generate_tiles(&hp->kd, hp->d_xr, hp->nr,
+                    hp->d_xc, hp->nc, (int)new_near, r0, c0, mm, nn, off,
+                    d_D);
+        if(info == STARSH_SUCCESS)
+            info = starsh_hip_memcpy_d2h(alloc_D, d_D, sizeof(double)*size_D);
+        starsh_hip_free(d_D);
+        for(bi = 0; bi < new_near && info == STARSH_SUCCESS; bi++)
+        {
+            int shape[2] = {mm[bi], nn[bi]};
+            info = array_from_buffer(near_D+bi, 2, shape, 'd', 'F',
+                    alloc_D+off[bi]);
+        }
+        free(r0); free(c0); free(off); free(mm); free(nn);
+        if(info != STARSH_SUCCESS)
+            return info;
+    }
+    hf->far_U = NULL;
+    hf->far_V = NULL;
+    hf->far_rank = NULL;
+    hf->alloc_U = NULL;
+    hf->alloc_V = NULL;
+    return starsh_blrm_new(matrix, F, far_rank, far_U, far_V, onfly, near_D,
+            alloc_U, alloc_V, alloc_D, '1');
+}
+
+/* Bytes of the plain pipeline's workspace for one tile. */
+static size_t tile_bytes(int m, int n, int k, int maxrank)
+{
+    size_t r = (size_t)(maxrank < k ? maxrank : k);
+    return sizeof(double)*((size_t)m*k+2*(size_t)n*k+4*(size_t)k*k+
+            17*(size_t)k+(size_t)(m+n)*r);
+}
+
+int starsh_blrm__drsdd_hip(STARSH_blrm **matrix, STARSH_blrf *format,
+        int maxrank, double tol, int onfly)
+{
+    double t_start = hip_driver_now();
+    if(matrix == NULL)
+    {
+        STARSH_ERROR("Invalid value of `matrix`");
+        return STARSH_WRONG_PARAMETER;
+    }
+    hip_problem hp;
+    int info = hip_driver_prepare(format, maxrank, tol, onfly, &hp);
+    if(info != STARSH_SUCCESS)
+        return info;
+    STARSH_blrf *F = format;
+    STARSH_cluster *RC = F->row_cluster, *CC = F->col_cluster;
+    STARSH_int nfar = F->nblocks_far;
+    hip_factors hf;
+    info = hip_driver_alloc_factors(F, maxrank, &hf);
+    starsh_hip_stats st;
+    memset(&st, 0, sizeof(st));
+    st.ntiles = nfar;
+    int *m = NULL, *n = NULL, *rk = NULL, *pp = NULL, *kc = NULL, *fl = NULL;
+    int64_t *r0 = NULL, *c0 = NULL, *uo = NULL, *vo = NULL;
+    if(info == STARSH_SUCCESS && nfar > 0)
+    {
+        m = malloc(sizeof(int)*nfar);
+        n = malloc(sizeof(int)*nfar);
+        rk = malloc(sizeof(int)*nfar);
+        pp = malloc(sizeof(int)*nfar);
+        kc = malloc(sizeof(int)*nfar);
+        fl = malloc(sizeof(int)*nfar);
+        r0 = malloc(sizeof(int64_t)*nfar);
+        c0 = malloc(sizeof(int64_t)*nfar);
+        uo = malloc(sizeof(int64_t)*nfar);
+        vo = malloc(sizeof(int64_t)*nfar);
+        if(!m || !n || !rk || !pp || !kc || !fl || !r0 || !c0 || !uo || !vo)
+            info = STARSH_MALLOC_ERROR;
+    }
+    for(STARSH_int bi = 0; info == STARSH_SUCCESS && bi < nfar; bi++)
+    {
+        STARSH_int i = F->block_far[2*bi], j = F->block_far[2*bi+1];
+        m[bi] = (int)RC->size[i];
+        n[bi] = (int)CC->size[j];
+        r0[bi] = RC->start[i];
+        c0[bi] = CC->start[j];
+        if(m[bi] != n[bi])
+            STARSH_WARNING("This was only tested on square tiles, error of "
+                    "approximation may be much higher, than demanded");
+    }
+    /* Chunks of tiles whose workspace fits the budget. */
+    STARSH_int b0 = 0;
+    double flops = 0;
+    while(info == STARSH_SUCCESS && b0 < nfar)
+    {
+        size_t bytes = 0;
+        STARSH_int b1 = b0;
+        while(b1 < nfar)
+        {
+            int mn = m[b1] < n[b1] ? m[b1] : n[b1];
+            int k = maxrank+hp.oversample < mn ? maxrank+hp.oversample : mn;
+            size_t tb = tile_bytes(m[b1], n[b1], k, maxrank);
+            if(b1 > b0 && bytes+tb > hip_driver_workspace_budget())
+                break;
+            bytes += tb;
+            flops += 4.0*m[b1]*(double)n[b1]*k;
+            b1++;
+        }
+        RsddInput in;
+        memset(&in, 0, sizeof(in));
+        in.ntiles = (int)(b1-b0);
+        in.m = m+b0;
+        in.n = n+b0;
+        in.kd = &hp.kd;
+        in.d_xr = hp.d_xr;
+        in.d_xc = hp.d_xc;
+        in.nr = hp.nr;
+        in.nc = hp.nc;
+        in.r0 = r0+b0;
+        in.c0 = c0+b0;
+        in.maxrank = maxrank;
+        in.oversample = hp.oversample;
+        in.tol = tol;
+        RsddOutput out;
+        memset(&out, 0, sizeof(out));
+        out.rank = rk+b0;
+        out.p = pp+b0;
+        out.kc = kc+b0;
+        out.flags = fl+b0;
+        out.uoff = uo+b0;
+        out.voff = vo+b0;
+        info = rsdd_batch(&in, &out);
+        if(info != STARSH_SUCCESS)
+            break;
+        st.device_seconds += out.device_seconds;
+        st.dominant_kernel_seconds += out.sketch_seconds;
+        st.dominant_kernel_launches += 1;
+        double tc = hip_driver_now();
+        for(STARSH_int bi = b0; bi < b1 && info == STARSH_SUCCESS; bi++)
+        {
+            hf.far_rank[bi] = rk[bi];
+            if(rk[bi] <= 0)
+                continue;
+            info = starsh_hip_memcpy_d2h(hf.alloc_U+hf.uoff[bi],
+                    out.d_uv+uo[bi], sizeof(double)*(size_t)m[bi]*rk[bi]);
+            if(info == STARSH_SUCCESS)
+                info = starsh_hip_memcpy_d2h(hf.alloc_V+hf.voff[bi],
+                        out.d_uv+vo[bi], sizeof(double)*(size_t)n[bi]*rk[bi]);
+        }
+        st.copy_seconds += hip_driver_now()-tc;
+        starsh_hip_free(out.d_uv);
+        b0 = b1;
+    }
+    if(info == STARSH_SUCCESS)
+    {
+        long long cnt = 0, sum = 0;
+        st.rank_min = 1 << 30;
+        st.rank_max = -1;
+        for(STARSH_int bi = 0; bi < nfar; bi++)
+            if(rk[bi] >= 0)
+            {
+                cnt++;
+                sum += rk[bi];
+                st.rank_min = rk[bi] < st.rank_min ? rk[bi] : st.rank_min;
+                st.rank_max = rk[bi] > st.rank_max ? rk[bi] : st.rank_max;
+            }
+        st.rank_avg = cnt ? (double)sum/cnt : 0;
+        if(cnt == 0)
+            st.rank_min = 0;
+        st.sketch_bytes_or_flops = flops/2;
+        st.flops_algorithmic = flops;
+    }
+    free(m); free(n); free(rk); free(pp); free(kc); free(fl);
+    free(r0); free(c0); free(uo); free(vo);
+    if(info == STARSH_SUCCESS)
+        info = hip_driver_finish(matrix, F, &hp, &hf, onfly, &st);
+    if(info != STARSH_SUCCESS)
+        hip_driver_free_factors(F, &hf);
+    hip_driver_release(&hp);
+    st.wall_seconds = hip_driver_now()-t_start;
+    hip_driver_set_stats(&st);
+    return info;
+}

@@ -1,0 +1,238 @@
+// Core of the randomized SVD: singular values and vectors of the small
+// matrix R2 (p x p, with B[:p, :]^T = Q2 R2), the rank by the reference rule
+// (src/backends/sequential/dense/dsvfr.c:16-47) and the factors W_r and
+// J_r from which U and V are formed.
+//
+// One-sided (Hestenes) Jacobi on the columns of R2 gives R2 J = W with
+// orthogonal columns; the column norms of W are the singular values, J holds
+// the left singular vectors of B, and Q2 W holds the right ones scaled by
+// the singular values:
+//   U = Q J[:, :r],  V = Q2 W[:, :r]   (V = S Vs^T as in drsdd.c:83-87).
+// J[:, :r] is recovered as R2^T W[:, :r] diag(sigma)^-2 instead of being
+// accumulated through the sweeps.
+#include <float.h>
+#include "pipeline.h"
+
+namespace {
+
+constexpr int CORE_THREADS = 256;
+constexpr int LDS_PMAX = 120;
+
+__device__ inline double group_sum(double v, int g)
+{
+    for(int o = g >> 1; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// W: p x p working matrix (ld = ldw), in LDS or global memory.
+__device__ void jacobi(double *W, int ldw, int p, int *sweeps_out)
+{
+    __shared__ int rotations;
+    const int pe = p+(p & 1);
+    const int npairs = pe/2;
+    int g = 64;
+    while(g > 1 && g*npairs > CORE_THREADS)
+        g >>= 1;
+    const int ngroups = CORE_THREADS/g;
+    const int grp = threadIdx.x/g, gl = threadIdx.x % g;
+    const double ctol = (double)p*DBL_EPSILON;
+    int sweep;
+    for(sweep = 0; sweep < 60; sweep++)
+    {
+        if(threadIdx.x == 0)
+            rotations = 0;
+        __syncthreads();
+        for(int round = 0; round < pe-1; round++)
+        {
+            for(int pi = grp; pi < npairs; pi += ngroups)
+            {
+                // Circle method: position 0 fixed, others rotate.
+                int x = pi, y = pe-1-pi;
+                int a = x == 0 ? 0 : 1+(x-1+round) % (pe-1);
+                int b = 1+(y-1+round) % (pe-1);
+                if(a > b)
+                {
+                    int tmp = a;
+                    a = b;
+                    b = tmp;
+                }
+                if(b >= p)
+                    continue;
+                double *wa = W+(int64_t)a*ldw, *wb = W+(int64_t)b*ldw;
+                double al = 0.0, be = 0.0, ga = 0.0;
+                for(int i = gl; i < p; i += g)
+                {
+                    double u = wa[i], v = wb[i];
+                    al += u*u;
+                    be += v*v;
+                    ga += u*v;
+                }
+                al = group_sum(al, g);
+                be = group_sum(be, g);
+                ga = group_sum(ga, g);
+                if(al > 0.0 && be > 0.0 && fabs(ga) > ctol*sqrt(al*be))
+                {
+                    double zeta = (be-al)/(2.0*ga);
+                    double t;
+                    if(fabs(zeta) > 1e150)
+                        t = 0.5/zeta;
+                    else
+                        t = (zeta >= 0.0 ? 1.0 : -1.0)/(fabs(zeta)+
+                                sqrt(1.0+zeta*zeta));
+                    double c = 1.0/sqrt(1.0+t*t), s = c*t;
+                    for(int i = gl; i < p; i += g)
+                    {
+                        double u = wa[i], v = wb[i];
+                        wa[i] = c*u-s*v;
+                        wb[i] = s*u+c*v;
+                    }
+                    if(gl == 0)
+                        atomicAdd(&rotations, 1);
+                }
+            }
+            __syncthreads();
+        }
+        if(rotations == 0)
+            break;
+        __syncthreads();
+    }
+    *sweeps_out = sweep;
+}
+
+__global__ __launch_bounds__(CORE_THREADS) void core_kernel(TileDesc *tiles,
+        double *ws, int maxrank, double tol)
+{
+    __shared__ double lds[LDS_PMAX*LDS_PMAX];
+    __shared__ double sig[1024];
+    __shared__ int perm[1024];
+    __shared__ int s_rank;
+    TileDesc &td = tiles[blockIdx.x];
+    const int p = td.p, k = td.k;
+    if(p <= 0)
+    {
+        if(threadIdx.x == 0)
+            td.rank = 0;
+        return;
+    }
+    const double *R2 = ws+td.r_off;
+    double *W;
+    int ldw;
+    if(p <= LDS_PMAX)
+    {
+        W = lds;
+        ldw = p;
+    }
+    else
+    {
+        W = ws+td.r_off+(int64_t)k*k;
+        ldw = p;
+    }
+    for(int e = threadIdx.x; e < p*p; e += blockDim.x)
+    {
+        int i = e % p, j = e / p;
+        W[(int64_t)j*ldw+i] = R2[(int64_t)j*k+i];
+    }
+    __syncthreads();
+    int sweeps;
+    jacobi(W, ldw, p, &sweeps);
+    __syncthreads();
+    // Singular values: column norms of W.
+    for(int j = threadIdx.x; j < p; j += blockDim.x)
+    {
+        double s = 0.0;
+        for(int i = 0; i < p; i++)
+            s += W[(int64_t)j*ldw+i]*W[(int64_t)j*ldw+i];
+        sig[j] = sqrt(s);
+    }
+    __syncthreads();
+    // Descending order, ties by index.
+    for(int j = threadIdx.x; j < p; j += blockDim.x)
+    {
+        int pos = 0;
+        double v = sig[j];
+        for(int l = 0; l < p; l++)
+            pos += sig[l] > v || (sig[l] == v && l < j);
+        perm[pos] = j;
+    }
+    __syncthreads();
+    double *S = ws+td.s_off;
+    for(int j = threadIdx.x; j < k; j += blockDim.x)
+    {
+        double v = 0.0;
+        if(j < p)
+            v = sig[perm[j]];
+        else if(j == p)
+            v = sqrt(td.tail2);
+        S[j] = v;
+    }
+    __syncthreads();
+    if(threadIdx.x == 0)
+    {
+        // dsvfr over the k values, in the reference's order.
+        int i;
+        double err_tol = 0;
+        for(i = 0; i < k; i++)
+            err_tol += S[i]*S[i];
+        int r;
+        if(err_tol == 0)
+            r = 0;
+        else
+        {
+            i = k;
+            err_tol *= tol*tol;
+            double tmp = S[k-1]*S[k-1];
+            while(i > 1 && err_tol >= tmp)
+            {
+                i--;
+                tmp += S[i-1]*S[i-1];
+            }
+            r = i;
+        }
+        if(r > p)
+            r = p;
+        if(r > maxrank)
+            r = -1;
+        s_rank = r;
+        td.rank = r;
+        td.flags |= sweeps >= 60 ? 4 : 0;
+    }
+    __syncthreads();
+    const int r = s_rank;
+    if(r <= 0)
+        return;
+    // W_r (p x r) then J_r = R2^T W_r diag(sigma)^-2 (p x r).
+    double *Wr = ws+td.w_off, *Jr = Wr+(int64_t)p*r;
+    for(int e = threadIdx.x; e < p*r; e += blockDim.x)
+    {
+        int i = e % p, c = e / p;
+        Wr[e] = W[(int64_t)perm[c]*ldw+i];
+    }
+    __syncthreads();
+    for(int e = threadIdx.x; e < p*r; e += blockDim.x)
+    {
+        int i = e % p, c = e / p;
+        const double *wc = W+(int64_t)perm[c]*ldw;
+        const double *ri = R2+(int64_t)i*k;
+        double s = 0.0;
+        for(int l = 0; l < p; l++)
+            s += ri[l]*wc[l];
+        double sg = sig[perm[c]];
+        Jr[e] = s/(sg*sg);
+    }
+}
+
+} // namespace
+
+int pipeline_core(TileDesc *d_tiles, int ntiles, double *ws, int maxrank,
+        double tol, int pmax, hipStream_t s)
+{
+    if(ntiles <= 0)
+        return STARSH_SUCCESS;
+    if(pmax > 1024)
+        return STARSH_WRONG_PARAMETER;
+    hipLaunchKernelGGL(core_kernel, dim3(ntiles), dim3(CORE_THREADS), 0, s,
+            d_tiles, ws, maxrank, tol);
+    HIP_TRY(hipGetLastError());
+    return STARSH_SUCCESS;
+}

@@ -1,0 +1,303 @@
+// Batched FP64 GEMMs of the randomized-SVD pipeline on MFMA
+// (v_mfma_f64_16x16x4f64).  One launch covers every tile of a batch:
+// grid.y is the tile, grid.x the 64 x 64 output blocks of the largest tile;
+// blocks outside a tile's own output return at once.
+//
+// Modes (C = alpha*op(A)*op(B) + beta*C):
+//   SKETCH_GEN  Y[m x k]  = A[m x n] Omega[n x k], A generated from points
+//   SKETCH_MEM  same with A explicit in memory
+//   PROJ_GEN    Bt[n x kc] = A^T[n x m] Q[m x kc], A generated
+//   PROJ_MEM    same with A explicit
+//   COEF        C[j0 x jb] = Q[:, :j0]^T W, W = Q[:, j0:j0+jb]
+//   UPDATE      W -= Q[:, :j0] C
+//   R2          R2[p x p] = Q2^T Bt[:, :p]
+//   FORM_U      U[m x r] = Q[:, :p] J_r
+//   FORM_V      V[n x r] = Q2 W_r
+#include "hip_common.h"
+#include "pipeline.h"
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int BM = 64, BN = 64, BK = 16, LDA_S = BM+2, LDB_S = BN+2;
+
+struct Op
+{
+    const double *A;   // memory operand A (unused when generated)
+    const double *B;
+    double *C;
+    int M, N, K;
+    int lda, ldb, ldc;
+    double alpha, beta;
+    int64_t gr0, gc0;  // generated operand: row / column point offsets
+    bool valid;
+};
+
+template <int MODE>
+__device__ inline Op setup(const TileDesc &t, const GemmArgs &g)
+{
+    Op o;
+    o.valid = true;
+    o.alpha = 1.0;
+    o.beta = 0.0;
+    o.gr0 = t.r0;
+    o.gc0 = t.c0;
+    double *ws = g.ws;
+    const bool q2 = g.which != 0;
+    const int rows = q2 ? t.n : t.m;
+    const int64_t qoff = q2 ? t.q2_off : t.q_off;
+    const int width = q2 ? t.p : t.k;
+    switch(MODE)
+    {
+        case GM_SKETCH_GEN:
+        case GM_SKETCH_MEM:
+            o.M = t.m; o.N = t.k; o.K = t.n;
+            o.A = g.abase+t.a_off; o.lda = t.m;
+            o.B = ws+t.om_off; o.ldb = t.n;
+            o.C = ws+t.q_off; o.ldc = t.m;
+            break;
+        case GM_PROJ_GEN:
+        case GM_PROJ_MEM:
+            o.M = t.n; o.N = t.kc; o.K = t.m;
+            o.A = g.abase+t.a_off; o.lda = t.m;
+            o.B = ws+t.q_off; o.ldb = t.m;
+            o.C = ws+t.bt_off; o.ldc = t.n;
+            break;
+        case GM_COEF:
+        {
+            int jb = width-g.j0 < g.jb ? width-g.j0 : g.jb;
+            o.M = g.j0; o.N = jb; o.K = rows;
+            o.A = ws+qoff; o.lda = rows;
+            o.B = ws+qoff+(int64_t)g.j0*rows; o.ldb = rows;
+            o.C = ws+t.coef_off; o.ldc = t.k;
+            o.valid = jb > 0 && g.j0 > 0 && !(t.flags & (q2 ? 2 : 1));
+            break;
+        }
+        case GM_UPDATE:
+        {
+            int jb = width-g.j0 < g.jb ? width-g.j0 : g.jb;
+            o.M = rows; o.N = jb; o.K = g.j0;
+            o.A = ws+qoff; o.lda = rows;
+            o.B = ws+t.coef_off; o.ldb = t.k;
+            o.C = ws+qoff+(int64_t)g.j0*rows; o.ldc = rows;
+            o.alpha = -1.0; o.beta = 1.0;
+            o.valid = jb > 0 && g.j0 > 0 && !(t.flags & (q2 ? 2 : 1));
+            break;
+        }
+        case GM_R2:
+            o.M = t.p; o.N = t.p; o.K = t.n;
+            o.A = ws+t.q2_off; o.lda = t.n;
+            o.B = ws+t.bt_off; o.ldb = t.n;
+            o.C = ws+t.r_off; o.ldc = t.k;
+            o.valid = t.p > 0;
+            break;
+        case GM_FORM_U:
+            o.M = t.m; o.N = t.rank; o.K = t.p;
+            o.A = ws+t.q_off; o.lda = t.m;
+            o.B = ws+t.w_off+(int64_t)t.p*(t.rank > 0 ? t.rank : 0);
+            o.ldb = t.p;
+            o.C = g.out+t.u_off; o.ldc = t.m;
+            o.valid = t.rank > 0;
+            break;
+        default: // GM_FORM_V
+            o.M = t.n; o.N = t.rank; o.K = t.p;
+            o.A = ws+t.q2_off; o.lda = t.n;
+            o.B = ws+t.w_off; o.ldb = t.p;
+            o.C = g.out+t.v_off; o.ldc = t.n;
+            o.valid = t.rank > 0;
+            break;
+    }
+    return o;
+}
+
+// A operand layouts: 0 = column-major (element (i,k) at A[k*lda+i]),
+// 1 = transposed storage (element (i,k) at A[i*lda+k]), 2 = generated A,
+// 3 = generated A^T.
+template <int MODE> struct ModeTraits;
+template <> struct ModeTraits<GM_SKETCH_GEN> { static constexpr int a = 2; };
+template <> struct ModeTraits<GM_SKETCH_MEM> { static constexpr int a = 0; };
+template <> struct ModeTraits<GM_PROJ_GEN> { static constexpr int a = 3; };
+template <> struct ModeTraits<GM_PROJ_MEM> { static constexpr int a = 1; };
+template <> struct ModeTraits<GM_COEF> { static constexpr int a = 1; };
+template <> struct ModeTraits<GM_UPDATE> { static constexpr int a = 0; };
+template <> struct ModeTraits<GM_R2> { static constexpr int a = 1; };
+template <> struct ModeTraits<GM_FORM_U> { static constexpr int a = 0; };
+template <> struct ModeTraits<GM_FORM_V> { static constexpr int a = 0; };
+
+template <int MODE, int NDIM>
+__global__ __launch_bounds__(256) void gemm_kernel(const TileDesc *tiles,
+        GemmArgs g)
+{
+    constexpr int AL = ModeTraits<MODE>::a;
+    const TileDesc t = tiles[blockIdx.y];
+    const Op o = setup<MODE>(t, g);
+    if(!o.valid || o.M <= 0 || o.N <= 0)
+        return;
+    const int nbm = (o.M+BM-1)/BM;
+    const int bm = blockIdx.x % nbm, bn = blockIdx.x / nbm;
+    if(bn*BN >= o.N)
+        return;
+    const int row0 = bm*BM, col0 = bn*BN;
+
+    __shared__ double As[BK][LDA_S];
+    __shared__ double Bs[BK][LDB_S];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    double4_t acc[2][2];
+#pragma unroll
+    for(int a = 0; a < 2; a++)
+#pragma unroll
+        for(int b = 0; b < 2; b++)
+            acc[a][b] = double4_t{0.0, 0.0, 0.0, 0.0};
+
+    for(int k0 = 0; k0 < o.K; k0 += BK)
+    {
+        // ---- stage A (BM x BK) as As[k][i]
+        if(AL == 0 || AL == 2 || AL == 3)
+        {
+            const int i = tid & 63, gi = row0+i;
+#pragma unroll
+            for(int s = 0; s < 4; s++)
+            {
+                const int kk = (tid >> 6)+4*s, gk = k0+kk;
+                double v = 0.0;
+                if(gi < o.M && gk < o.K)
+                {
+                    if(AL == 0)
+                        v = o.A[(int64_t)gk*o.lda+gi];
+                    else if(AL == 2)
+                        v = kernel_value<NDIM>(g.kp, g.ps.xr, g.ps.nr,
+                                o.gr0+gi, g.ps.xc, g.ps.nc, o.gc0+gk);
+                    else
+                        v = kernel_value<NDIM>(g.kp, g.ps.xr, g.ps.nr,
+                                o.gr0+gk, g.ps.xc, g.ps.nc, o.gc0+gi);
+                }
+                As[kk][i] = v;
+            }
+        }
+        else
+        {
+            const int kk = tid & 15, gk = k0+kk;
+#pragma unroll
+            for(int s = 0; s < 4; s++)
+            {
+                const int i = (tid >> 4)+16*s, gi = row0+i;
+                double v = 0.0;
+                if(gi < o.M && gk < o.K)
+                    v = o.A[(int64_t)gi*o.lda+gk];
+                As[kk][i] = v;
+            }
+        }
+        // ---- stage B (BK x BN) as Bs[k][j]; B is column-major K x N
+        {
+            const int kk = tid & 15, gk = k0+kk;
+#pragma unroll
+            for(int s = 0; s < 4; s++)
+            {
+                const int j = (tid >> 4)+16*s, gj = col0+j;
+                double v = 0.0;
+                if(gj < o.N && gk < o.K)
+                    v = o.B[(int64_t)gj*o.ldb+gk];
+                Bs[kk][j] = v;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for(int k4 = 0; k4 < BK; k4 += 4)
+        {
+            const int kr = k4+(lane >> 4), c = lane & 15;
+            double a0 = As[kr][wm*32+c], a1 = As[kr][wm*32+16+c];
+            double b0 = Bs[kr][wn*32+c], b1 = Bs[kr][wn*32+16+c];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0,
+                    acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1,
+                    acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0,
+                    acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1,
+                    acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // ---- epilogue: lane holds rows 4*(lane/16)+v, column lane%16 of each
+    // 16 x 16 block.
+#pragma unroll
+    for(int a = 0; a < 2; a++)
+#pragma unroll
+        for(int b = 0; b < 2; b++)
+        {
+            const int gj = col0+wn*32+b*16+(lane & 15);
+            if(gj >= o.N)
+                continue;
+#pragma unroll
+            for(int v = 0; v < 4; v++)
+            {
+                const int gi = row0+wm*32+a*16+4*(lane >> 4)+v;
+                if(gi >= o.M)
+                    continue;
+                double *c = o.C+(int64_t)gj*o.ldc+gi;
+                double r = o.alpha*acc[a][b][v];
+                if(o.beta != 0.0)
+                    r += o.beta*(*c);
+                *c = r;
+            }
+        }
+}
+
+template <int MODE, int NDIM>
+int launch_ndim(const TileDesc *d_tiles, int ntiles, int mmax, int nmax,
+        const GemmArgs &g, hipStream_t s)
+{
+    int nbm = (mmax+BM-1)/BM, nbn = (nmax+BN-1)/BN;
+    if(nbm <= 0 || nbn <= 0 || ntiles <= 0)
+        return STARSH_SUCCESS;
+    hipLaunchKernelGGL((gemm_kernel<MODE, NDIM>), dim3(nbm*nbn, ntiles),
+            dim3(256), 0, s, d_tiles, g);
+    HIP_TRY(hipGetLastError());
+    return STARSH_SUCCESS;
+}
+
+template <int MODE>
+int launch(const TileDesc *d_tiles, int ntiles, int mmax, int nmax,
+        const GemmArgs &g, hipStream_t s)
+{
+    switch(g.kp.ndim)
+    {
+        case 1: return launch_ndim<MODE, 1>(d_tiles, ntiles, mmax, nmax, g, s);
+        case 2: return launch_ndim<MODE, 2>(d_tiles, ntiles, mmax, nmax, g, s);
+        case 3: return launch_ndim<MODE, 3>(d_tiles, ntiles, mmax, nmax, g, s);
+        default: return launch_ndim<MODE, 4>(d_tiles, ntiles, mmax, nmax, g,
+                         s);
+    }
+}
+
+} // namespace
+
+int pipeline_gemm(int mode, const TileDesc *d_tiles, int ntiles, int mmax,
+        int nmax, const GemmArgs &g, hipStream_t s)
+{
+    switch(mode)
+    {
+        case GM_SKETCH_GEN: return launch<GM_SKETCH_GEN>(d_tiles, ntiles, mmax,
+                                    nmax, g, s);
+        case GM_SKETCH_MEM: return launch_ndim<GM_SKETCH_MEM, 1>(d_tiles,
+                                    ntiles, mmax, nmax, g, s);
+        case GM_PROJ_GEN: return launch<GM_PROJ_GEN>(d_tiles, ntiles, mmax,
+                                  nmax, g, s);
+        case GM_PROJ_MEM: return launch_ndim<GM_PROJ_MEM, 1>(d_tiles, ntiles,
+                                  mmax, nmax, g, s);
+        case GM_COEF: return launch_ndim<GM_COEF, 1>(d_tiles, ntiles, mmax,
+                              nmax, g, s);
+        case GM_UPDATE: return launch_ndim<GM_UPDATE, 1>(d_tiles, ntiles, mmax,
+                                nmax, g, s);
+        case GM_R2: return launch_ndim<GM_R2, 1>(d_tiles, ntiles, mmax, nmax,
+                            g, s);
+        case GM_FORM_U: return launch_ndim<GM_FORM_U, 1>(d_tiles, ntiles, mmax,
+                                nmax, g, s);
+        case GM_FORM_V: return launch_ndim<GM_FORM_V, 1>(d_tiles, ntiles, mmax,
+                                nmax, g, s);
+        default: return STARSH_WRONG_PARAMETER;
+    }
+}

@@ -1,0 +1,158 @@
+// Shared definitions of the HIP backend: per-tile descriptor of a compression
+// batch, kernel-value functions, launch helpers.  gfx950 (MI355X) only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "starsh.h"
+#include "starsh-hip.h"
+
+#define HIP_TRY(expr) do { hipError_t e_ = (expr); if(e_ != hipSuccess) { \
+    fprintf(stderr, "STARSH HIP ERROR: %s at %s:%d\n", hipGetErrorString(e_), \
+            __FILE__, __LINE__); \
+    return e_ == hipErrorOutOfMemory ? STARSH_MALLOC_ERROR : \
+        STARSH_UNKNOWN_ERROR; } } while(0)
+
+// One far-field tile of a compression batch.  Offsets count doubles from
+// the base of the batch workspace; the device writes kc, p, rank, bnorm2,
+// tail2.
+struct TileDesc
+{
+    int m, n, k;       // tile rows, columns, sketch width
+    int gen;           // 1: A generated from points, 0: explicit A
+    int64_t r0, c0;    // first row / column point (cluster order)
+    int64_t a_off;     // explicit A (m x n, ld m) when gen == 0
+    int64_t om_off;    // test matrix Omega (n x k, ld n)
+    int64_t q_off;     // Y, then Q (m x k, ld m)
+    int64_t bt_off;    // Bt = A^T Q (n x k, ld n)
+    int64_t q2_off;    // Q2 (n x k, ld n)
+    int64_t r_off;     // R2 (k x k, ld k)
+    int64_t w_off;     // W_r (p x r, ld p), then J_r after it
+    int64_t s_off;     // singular values, k entries
+    int64_t coef_off;  // projection coefficients (k x 32, ld k)
+    int64_t u_off;     // U (m x maxrank, ld m)
+    int64_t v_off;     // V (n x maxrank, ld n)
+    int kc;            // kept (numerically independent) sketch columns
+    int p;             // size of the core problem after truncation
+    int rank;          // chosen rank, -1 for a false far-field tile
+    int flags;
+    double bnorm2;     // |B|_F^2
+    double tail2;      // squared norm of the truncated rows of B
+};
+
+// Kernel parameters in the form the device evaluates them.  Derived on the
+// host with the same operations as the reference kernels.
+struct KernelParams
+{
+    int family;
+    int ndim;
+    double bexp;   // -beta                 (kernel_exp.c:74)
+    double bsqr;   // -2*beta*beta          (kernel_sqrexp.c:75)
+    double theta;  // sqrt(2*nu)/beta       (kernel_matern.c:82)
+    double beta;
+    double nu;
+    double sigma;
+    double noise;
+    double wave_k;
+    double diag;
+};
+
+static inline KernelParams make_params(const starsh_hip_kernel_desc *kd)
+{
+    KernelParams p;
+    p.family = kd->family;
+    p.ndim = kd->ndim;
+    p.bexp = -kd->beta;
+    p.bsqr = -2*kd->beta*kd->beta;
+    p.theta = sqrt(2*kd->nu)/kd->beta;
+    p.beta = kd->beta;
+    p.nu = kd->nu;
+    p.sigma = kd->sigma;
+    p.noise = kd->noise;
+    p.wave_k = kd->wave_k;
+    p.diag = kd->diag;
+    return p;
+}
+
+__device__ inline double matern_closed(double nu, double x)
+{
+    double e = exp(-x);
+    if(nu == 0.5)
+        return e;
+    if(nu == 1.5)
+        return (1.0+x)*e;
+    return (1.0+x+x*x/3.0)*e;
+}
+
+// Entry A(i, j) = f(|x_i - y_j|) in FP64 with the reference's operation
+// order (no contraction into fused multiply-adds).
+template <int NDIM>
+__device__ inline double kernel_value(const KernelParams &kp,
+        const double *__restrict__ xr, int64_t nr, int64_t i,
+        const double *__restrict__ xc, int64_t nc, int64_t j)
+{
+#pragma clang fp contract(off)
+    double dist = 0.0;
+#pragma unroll
+    for(int d = 0; d < NDIM; d++)
+    {
+        double t = xr[d*nr+i]-xc[d*nc+j];
+        dist += t*t;
+    }
+    double v;
+    switch(kp.family)
+    {
+        case STARSH_HIP_EXP:
+            dist = sqrt(dist)/kp.bexp;
+            v = dist == 0 ? kp.sigma+kp.noise : kp.sigma*exp(dist);
+            break;
+        case STARSH_HIP_SQREXP:
+            dist = dist/kp.bsqr;
+            v = dist == 0 ? kp.sigma+kp.noise : kp.sigma*exp(dist);
+            break;
+        case STARSH_HIP_MATERN:
+            dist = sqrt(dist)*kp.theta;
+            v = dist == 0 ? kp.sigma+kp.noise :
+                kp.sigma*matern_closed(kp.nu, dist);
+            break;
+        case STARSH_HIP_MATERN2:
+            dist = sqrt(dist)/kp.beta;
+            v = dist == 0 ? kp.sigma+kp.noise :
+                kp.sigma*matern_closed(kp.nu, dist);
+            break;
+        case STARSH_HIP_COULOMB:
+            v = dist == 0 ? 0.0 : 1.0/sqrt(dist);
+            break;
+        case STARSH_HIP_SIN:
+            if(dist == 0)
+                v = kp.diag;
+            else
+            {
+                dist = sqrt(dist);
+                v = sin(kp.wave_k*dist)/dist;
+            }
+            break;
+        default:
+            if(dist == 0)
+                v = kp.diag;
+            else
+            {
+                dist = sqrt(dist);
+                v = cos(kp.wave_k*dist)/dist;
+            }
+            break;
+    }
+    return v;
+}
+
+// Points of a batch: row and column point sets in cluster order, SoA.
+struct PointSets
+{
+    const double *xr;
+    int64_t nr;
+    const double *xc;
+    int64_t nc;
+};
+
+// Library stream (created by starsh_hip_init).
+extern "C" hipStream_t starsh_hip_stream(void);

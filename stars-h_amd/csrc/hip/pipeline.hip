@@ -1,0 +1,234 @@
+// Orchestration of the plain FP64 randomized-SVD pipeline over a batch of
+// tiles (src/backends/sequential/dense/drsdd.c:16-93 applied to every tile
+// of src/backends/openmp/blrm/drsdd.c:94-145):
+//   Y = A Omega; Q = orth(Y); Bt = A^T Q; truncate; Bt[:, :p] = Q2 R2;
+//   Jacobi SVD of R2; rank; U = Q J_r; V = Q2 W_r.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <map>
+#include <vector>
+#include <utility>
+#include "pipeline.h"
+#include "rsdd.h"
+
+namespace {
+
+// Deflation thresholds of the Gram-Schmidt passes (relative to the column
+// norm before projection, and to 1 for the second pass).
+constexpr double TAU_FIRST = 1e-13;
+constexpr double TAU_SECOND = 1e-8;
+// Truncation of B's rows: discarded tail at most (ETA*tol)^2 |B|_F^2.
+constexpr double ETA = 1e-5;
+
+struct DeviceBuf
+{
+    void *p = nullptr;
+    ~DeviceBuf() { if(p) (void)hipFree(p); }
+    int alloc(size_t bytes)
+    {
+        if(hipMalloc(&p, bytes ? bytes : 1) != hipSuccess)
+        {
+            p = nullptr;
+            fprintf(stderr, "STARSH HIP ERROR: hipMalloc(%zu) failed\n",
+                    bytes);
+            return STARSH_MALLOC_ERROR;
+        }
+        return STARSH_SUCCESS;
+    }
+};
+
+#define TRY(expr) do { int i_ = (expr); if(i_ != STARSH_SUCCESS) return i_; } \
+    while(0)
+
+int bcgs2(TileDesc *d_tiles, int ntiles, int which, double *ws,
+        const double *d_ref, int kmax, int width_max, int rows_max,
+        int *d_keep, GemmArgs g, hipStream_t s)
+{
+    const int pb = rows_max <= 2048 ? 16 : 8;
+    g.which = which;
+    g.jb = pb;
+    for(int j0 = 0; j0 < width_max; j0 += pb)
+    {
+        g.j0 = j0;
+        for(int pass = 0; pass < 2; pass++)
+        {
+            if(j0 > 0)
+            {
+                TRY(pipeline_gemm(GM_COEF, d_tiles, ntiles, j0, pb, g, s));
+                TRY(pipeline_gemm(GM_UPDATE, d_tiles, ntiles, rows_max, pb, g,
+                            s));
+            }
+            TRY(pipeline_panel(d_tiles, ntiles, which, ws, j0, pb,
+                        pass == 0 ? d_ref : nullptr, kmax,
+                        pass == 0 ? TAU_FIRST : TAU_SECOND, d_keep, rows_max,
+                        s));
+        }
+    }
+    return STARSH_SUCCESS;
+}
+
+} // namespace
+
+// Runs the pipeline on ntiles tiles.  Either the kernel (kd, points) or
+// explicit tiles (d_a, a_off) define the tile contents.  Results: ranks in
+// rank_out, U/V in device memory (*d_uv, offsets in uoff/voff, allocated
+// here and owned by the caller), singular values (k per tile) copied to
+// S_out + soff[t] when S_out is not NULL.
+int rsdd_batch(const RsddInput *in, RsddOutput *out)
+{
+    hipStream_t s = starsh_hip_stream();
+    const int nt = in->ntiles;
+    out->d_uv = nullptr;
+    if(nt <= 0)
+        return STARSH_SUCCESS;
+    std::vector<TileDesc> h(nt);
+    std::map<std::pair<int, int>, int64_t> omegas;
+    int64_t off = 0;
+    int kmax = 0, mmax = 0, nmax = 0;
+    for(int t = 0; t < nt; t++)
+    {
+        TileDesc &d = h[t];
+        memset(&d, 0, sizeof(d));
+        d.m = in->m[t];
+        d.n = in->n[t];
+        int mn = d.m < d.n ? d.m : d.n;
+        d.k = in->maxrank+in->oversample;
+        if(d.k > mn)
+            d.k = mn;
+        d.gen = in->d_a == nullptr;
+        d.r0 = in->r0 ? in->r0[t] : 0;
+        d.c0 = in->c0 ? in->c0[t] : 0;
+        d.a_off = in->a_off ? in->a_off[t] : 0;
+        kmax = d.k > kmax ? d.k : kmax;
+        mmax = d.m > mmax ? d.m : mmax;
+        nmax = d.n > nmax ? d.n : nmax;
+    }
+    // Workspace layout: all Omega first, then per-tile regions.
+    for(int t = 0; t < nt; t++)
+    {
+        auto key = std::make_pair(h[t].n, h[t].k);
+        if(omegas.find(key) == omegas.end())
+        {
+            omegas[key] = off;
+            off += (int64_t)h[t].n*h[t].k;
+        }
+        h[t].om_off = omegas[key];
+    }
+    int64_t uvoff = 0;
+    for(int t = 0; t < nt; t++)
+    {
+        TileDesc &d = h[t];
+        int64_t m = d.m, n = d.n, k = d.k;
+        int64_t rmax = in->maxrank < k ? in->maxrank : k;
+        d.q_off = off; off += m*k;
+        d.bt_off = off; off += n*k;
+        d.q2_off = off; off += n*k;
+        d.r_off = off; off += 2*k*k;
+        d.w_off = off; off += 2*k*k;
+        d.s_off = off; off += k;
+        d.coef_off = off; off += k*16;
+        d.u_off = uvoff; uvoff += m*rmax;
+        d.v_off = uvoff; uvoff += n*rmax;
+        out->uoff[t] = d.u_off;
+        out->voff[t] = d.v_off;
+    }
+    if(in->rows_out)
+        for(int t = 0; t < nt; t++)
+            in->rows_out[t] = h[t].k;
+    DeviceBuf ws, tiles, keep, norms, uv;
+    TRY(ws.alloc(sizeof(double)*off));
+    TRY(tiles.alloc(sizeof(TileDesc)*nt));
+    TRY(keep.alloc(sizeof(int)*(size_t)nt*kmax));
+    TRY(norms.alloc(sizeof(double)*(size_t)nt*kmax));
+    TRY(uv.alloc(sizeof(double)*uvoff));
+    TileDesc *d_tiles = (TileDesc *)tiles.p;
+    double *d_ws = (double *)ws.p;
+    HIP_TRY(hipMemcpyAsync(d_tiles, h.data(), sizeof(TileDesc)*nt,
+                hipMemcpyHostToDevice, s));
+    for(auto &o : omegas)
+        TRY(starsh_hip_omega(d_ws+o.second, o.first.first, o.first.second));
+
+    GemmArgs g;
+    memset(&g, 0, sizeof(g));
+    g.ws = d_ws;
+    g.out = (double *)uv.p;
+    g.abase = in->d_a;
+    if(in->kd)
+        g.kp = make_params(in->kd);
+    else
+        g.kp.ndim = 1;
+    g.ps = PointSets{in->d_xr, in->nr, in->d_xc, in->nc};
+
+    hipEvent_t ev[4];
+    for(int e = 0; e < 4; e++)
+        HIP_TRY(hipEventCreate(&ev[e]));
+    HIP_TRY(hipEventRecord(ev[0], s));
+    // 1. sketch
+    HIP_TRY(hipEventRecord(ev[2], s));
+    TRY(pipeline_gemm(in->d_a ? GM_SKETCH_MEM : GM_SKETCH_GEN, d_tiles, nt,
+                mmax, kmax, g, s));
+    HIP_TRY(hipEventRecord(ev[3], s));
+    // 2. orthonormal basis with deflation, kept columns to the front
+    TRY(pipeline_colnorm(d_tiles, nt, 0, d_ws, (double *)norms.p, kmax, s));
+    TRY(bcgs2(d_tiles, nt, 0, d_ws, (double *)norms.p, kmax, kmax, mmax,
+                (int *)keep.p, g, s));
+    TRY(pipeline_compact(d_tiles, nt, d_ws, (int *)keep.p, kmax, s));
+    // 3. projection Bt = A^T Q
+    TRY(pipeline_gemm(in->d_a ? GM_PROJ_MEM : GM_PROJ_GEN, d_tiles, nt, nmax,
+                kmax, g, s));
+    // 4. truncation of B's rows
+    TRY(pipeline_colnorm(d_tiles, nt, 2, d_ws, (double *)norms.p, kmax, s));
+    TRY(pipeline_truncate(d_tiles, nt, d_ws, (double *)norms.p, kmax,
+                ETA*in->tol, nmax, s));
+    HIP_TRY(hipMemcpyAsync(h.data(), d_tiles, sizeof(TileDesc)*nt,
+                hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    int pmax = 0;
+    for(int t = 0; t < nt; t++)
+        pmax = h[t].p > pmax ? h[t].p : pmax;
+    // 5. Bt[:, :p] = Q2 R2
+    if(pmax > 0)
+    {
+        TRY(bcgs2(d_tiles, nt, 1, d_ws, (double *)norms.p, kmax, pmax, nmax,
+                    (int *)keep.p, g, s));
+        TRY(pipeline_gemm(GM_R2, d_tiles, nt, pmax, pmax, g, s));
+    }
+    // 6. core SVD and rank
+    TRY(pipeline_core(d_tiles, nt, d_ws, in->maxrank, in->tol, pmax, s));
+    HIP_TRY(hipMemcpyAsync(h.data(), d_tiles, sizeof(TileDesc)*nt,
+                hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    int rmax = 0;
+    for(int t = 0; t < nt; t++)
+        rmax = h[t].rank > rmax ? h[t].rank : rmax;
+    // 7. factors
+    if(rmax > 0)
+    {
+        TRY(pipeline_gemm(GM_FORM_U, d_tiles, nt, mmax, rmax, g, s));
+        TRY(pipeline_gemm(GM_FORM_V, d_tiles, nt, nmax, rmax, g, s));
+    }
+    HIP_TRY(hipEventRecord(ev[1], s));
+    if(in->S_out)
+        for(int t = 0; t < nt; t++)
+            HIP_TRY(hipMemcpyAsync(in->S_out+in->soff[t], d_ws+h[t].s_off,
+                        sizeof(double)*h[t].k, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    float ms = 0, ms_sk = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    HIP_TRY(hipEventElapsedTime(&ms_sk, ev[2], ev[3]));
+    for(int e = 0; e < 4; e++)
+        (void)hipEventDestroy(ev[e]);
+    out->device_seconds = ms*1e-3;
+    out->sketch_seconds = ms_sk*1e-3;
+    for(int t = 0; t < nt; t++)
+    {
+        out->rank[t] = h[t].rank;
+        out->p[t] = h[t].p;
+        out->kc[t] = h[t].kc;
+        out->flags[t] = h[t].flags;
+    }
+    out->d_uv = (double *)uv.p;
+    uv.p = nullptr;
+    return STARSH_SUCCESS;
+}

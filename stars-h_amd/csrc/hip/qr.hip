@@ -1,0 +1,316 @@
+// Orthogonalisation stages of the randomized SVD: block classical
+// Gram-Schmidt applied twice (BCGS2) with column deflation, compaction of the
+// kept columns, and truncation of the projected matrix.
+//
+// The reference factors the sketch with Householder QR (dgeqrf + dorgqr,
+// src/backends/sequential/dense/drsdd.c:64-67).  The sketch of a smooth
+// kernel tile is numerically rank deficient far below the sketch width, and
+// plain block Gram-Schmidt loses orthogonality on such columns.  Columns
+// whose residual after projection falls to tau times their original norm
+// are therefore dropped (set to zero): they carry no information above
+// rounding, and leaving them out changes the singular values of B = Q^T A
+// only at the level of tau*|A|.  Once a whole panel is dropped, the range
+// of the sketch is exhausted and the remaining panels of that tile are
+// skipped.
+#include "pipeline.h"
+
+namespace {
+
+constexpr int PANEL_THREADS = 1024;
+constexpr int NWAVES = PANEL_THREADS/64;
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for(int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Column norms squared: which = 0 -> Q buffer (m x k), 2 -> Bt (n x kc).
+__global__ __launch_bounds__(256) void colnorm_kernel(const TileDesc *tiles,
+        int which, const double *ws, double *out, int kmax)
+{
+    const TileDesc t = tiles[blockIdx.y];
+    const int rows = which == 0 ? t.m : t.n;
+    const int cols = which == 0 ? t.k : t.kc;
+    const double *base = ws+(which == 0 ? t.q_off : t.bt_off);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for(int c = blockIdx.x*4+wave; c < cols; c += gridDim.x*4)
+    {
+        const double *col = base+(int64_t)c*rows;
+        double s = 0.0;
+        for(int i = lane; i < rows; i += 64)
+            s += col[i]*col[i];
+        s = wave_sum(s);
+        if(lane == 0)
+            out[(int64_t)blockIdx.y*kmax+c] = s;
+    }
+}
+
+// Reduces cnt per-thread values (cnt <= PB) over the block; result in res.
+template <int PB>
+__device__ inline void block_sum(double (&v)[PB], int cnt, double *red,
+        double *res)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for(int l = 0; l < PB; l++)
+        if(l < cnt)
+        {
+            double s = wave_sum(v[l]);
+            if(lane == 0)
+                red[wave*PB+l] = s;
+        }
+    __syncthreads();
+    if(threadIdx.x < cnt)
+    {
+        double s = 0.0;
+#pragma unroll
+        for(int w = 0; w < NWAVES; w++)
+            s += red[w*PB+threadIdx.x];
+        res[threadIdx.x] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for(int l = 0; l < PB; l++)
+        if(l < cnt)
+            v[l] = res[l];
+    __syncthreads();
+}
+
+// Intra-panel CGS2 with deflation.  The panel (rows x PB) lives in
+// registers: thread r holds rows r, r+1024, ... (RPT of them).
+template <int RPT, int PB>
+__global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(
+        TileDesc *tiles, int which, double *ws, int j0, const double *ref,
+        int kmax, double tau2, int *keep, int first_pass)
+{
+    __shared__ double red[NWAVES*PB];
+    __shared__ double res[PB];
+    TileDesc &td = tiles[blockIdx.x];
+    const bool q2 = which != 0;
+    const int rows = q2 ? td.n : td.m;
+    const int width = q2 ? td.p : td.k;
+    if(j0 >= width)
+        return;
+    const int jb = width-j0 < PB ? width-j0 : PB;
+    int *kp = keep+(int64_t)blockIdx.x*kmax+j0;
+    if(td.flags & (q2 ? 2 : 1))
+    {
+        // Range already exhausted: every remaining column is dropped.
+        if(threadIdx.x < jb)
+            kp[threadIdx.x] = 0;
+        return;
+    }
+    double *W = ws+(q2 ? td.q2_off : td.q_off)+(int64_t)j0*rows;
+    double w[RPT][PB];
+#pragma unroll
+    for(int r = 0; r < RPT; r++)
+    {
+        const int i = threadIdx.x+r*PANEL_THREADS;
+#pragma unroll
+        for(int c = 0; c < PB; c++)
+            w[r][c] = (i < rows && c < jb) ? W[(int64_t)c*rows+i] : 0.0;
+    }
+    int ndropped = 0;
+#pragma unroll
+    for(int c = 0; c < PB; c++)
+    {
+        if(c < jb)
+        {
+#pragma unroll
+            for(int pass = 0; pass < 2; pass++)
+            {
+                if(c > 0)
+                {
+                    double d[PB];
+#pragma unroll
+                    for(int l = 0; l < PB; l++)
+                    {
+                        d[l] = 0.0;
+                        if(l < c)
+#pragma unroll
+                            for(int r = 0; r < RPT; r++)
+                                d[l] += w[r][l]*w[r][c];
+                    }
+                    block_sum<PB>(d, c, red, res);
+#pragma unroll
+                    for(int l = 0; l < PB; l++)
+                        if(l < c)
+#pragma unroll
+                            for(int r = 0; r < RPT; r++)
+                                w[r][c] -= d[l]*w[r][l];
+                }
+            }
+            double nv[PB];
+#pragma unroll
+            for(int l = 0; l < PB; l++)
+                nv[l] = 0.0;
+#pragma unroll
+            for(int r = 0; r < RPT; r++)
+                nv[0] += w[r][c]*w[r][c];
+            block_sum<PB>(nv, 1, red, res);
+            const double n2 = nv[0];
+            const double r2 = ref ? ref[(int64_t)blockIdx.x*kmax+j0+c] : 1.0;
+            const bool drop = !(n2 > tau2*r2) || n2 == 0.0;
+            const double inv = drop ? 0.0 : 1.0/sqrt(n2);
+#pragma unroll
+            for(int r = 0; r < RPT; r++)
+                w[r][c] *= inv;
+            if(drop)
+                ndropped++;
+            if(threadIdx.x == 0)
+            {
+                if(first_pass)
+                    kp[c] = drop ? 0 : 1;
+                else if(drop)
+                    kp[c] = 0;
+            }
+        }
+    }
+#pragma unroll
+    for(int r = 0; r < RPT; r++)
+    {
+        const int i = threadIdx.x+r*PANEL_THREADS;
+        if(i < rows)
+#pragma unroll
+            for(int c = 0; c < PB; c++)
+                if(c < jb)
+                    W[(int64_t)c*rows+i] = w[r][c];
+    }
+    if(first_pass && ndropped == jb && threadIdx.x == 0)
+        td.flags |= q2 ? 2 : 1;
+}
+
+// Moves kept columns of Q to the front, in order; sets kc.
+__global__ __launch_bounds__(256) void compact_kernel(TileDesc *tiles,
+        double *ws, const int *keep, int kmax)
+{
+    TileDesc &td = tiles[blockIdx.x];
+    const int *kp = keep+(int64_t)blockIdx.x*kmax;
+    double *Q = ws+td.q_off;
+    const int m = td.m;
+    int c = 0;
+    for(int j = 0; j < td.k; j++)
+    {
+        if(!kp[j])
+            continue;
+        if(c != j)
+        {
+            for(int i = threadIdx.x; i < m; i += blockDim.x)
+                Q[(int64_t)c*m+i] = Q[(int64_t)j*m+i];
+            __syncthreads();
+        }
+        c++;
+    }
+    if(threadIdx.x == 0)
+        td.kc = c;
+}
+
+// |B|_F^2, truncation point p (smallest p >= 1 whose row tail of B is at
+// most (eta*tol)^2 |B|_F^2) and the tail; copies Bt[:, :p] into Q2.
+__global__ __launch_bounds__(256) void truncate_kernel(TileDesc *tiles,
+        double *ws, const double *bnorm2, int kmax, double eta_tol)
+{
+    TileDesc &td = tiles[blockIdx.x];
+    const double *nb = bnorm2+(int64_t)blockIdx.x*kmax;
+    __shared__ int sp;
+    if(threadIdx.x == 0)
+    {
+        double total = 0.0;
+        for(int c = 0; c < td.kc; c++)
+            total += nb[c];
+        double limit = eta_tol*eta_tol*total;
+        int p = td.kc;
+        double tail = 0.0;
+        while(p > 1 && tail+nb[p-1] <= limit)
+        {
+            tail += nb[p-1];
+            p--;
+        }
+        if(total == 0.0)
+        {
+            p = 0;
+            tail = 0.0;
+        }
+        td.p = p;
+        td.tail2 = tail;
+        td.bnorm2 = total;
+        sp = p;
+    }
+    __syncthreads();
+    const int p = sp;
+    const int64_t cnt = (int64_t)td.n*p;
+    const double *src = ws+td.bt_off;
+    double *dst = ws+td.q2_off;
+    for(int64_t e = threadIdx.x; e < cnt; e += blockDim.x)
+        dst[e] = src[e];
+}
+
+} // namespace
+
+int pipeline_colnorm(const TileDesc *d_tiles, int ntiles, int which,
+        double *ws, double *d_norm, int kmax, hipStream_t s)
+{
+    if(ntiles <= 0)
+        return STARSH_SUCCESS;
+    int gx = (kmax+3)/4;
+    if(gx > 32)
+        gx = 32;
+    hipLaunchKernelGGL(colnorm_kernel, dim3(gx, ntiles), dim3(256), 0, s,
+            d_tiles, which, ws, d_norm, kmax);
+    HIP_TRY(hipGetLastError());
+    return STARSH_SUCCESS;
+}
+
+int pipeline_panel(const TileDesc *d_tiles, int ntiles, int which,
+        double *ws, int j0, int jb, const double *d_ref, int kmax,
+        double tau, int *d_keep, int rows_max, hipStream_t s)
+{
+    if(ntiles <= 0)
+        return STARSH_SUCCESS;
+    TileDesc *tiles = const_cast<TileDesc *>(d_tiles);
+    const int first = d_ref != nullptr;
+    const double tau2 = tau*tau;
+    if(rows_max <= 1024 && jb == 16)
+        hipLaunchKernelGGL((panel_kernel<1, 16>), dim3(ntiles),
+                dim3(PANEL_THREADS), 0, s, tiles, which, ws, j0, d_ref, kmax,
+                tau2, d_keep, first);
+    else if(rows_max <= 2048 && jb == 16)
+        hipLaunchKernelGGL((panel_kernel<2, 16>), dim3(ntiles),
+                dim3(PANEL_THREADS), 0, s, tiles, which, ws, j0, d_ref, kmax,
+                tau2, d_keep, first);
+    else if(rows_max <= 4096 && jb == 8)
+        hipLaunchKernelGGL((panel_kernel<4, 8>), dim3(ntiles),
+                dim3(PANEL_THREADS), 0, s, tiles, which, ws, j0, d_ref, kmax,
+                tau2, d_keep, first);
+    else
+        return STARSH_WRONG_PARAMETER;
+    HIP_TRY(hipGetLastError());
+    return STARSH_SUCCESS;
+}
+
+int pipeline_compact(TileDesc *d_tiles, int ntiles, double *ws,
+        const int *d_keep, int kmax, hipStream_t s)
+{
+    if(ntiles <= 0)
+        return STARSH_SUCCESS;
+    hipLaunchKernelGGL(compact_kernel, dim3(ntiles), dim3(256), 0, s,
+            d_tiles, ws, d_keep, kmax);
+    HIP_TRY(hipGetLastError());
+    return STARSH_SUCCESS;
+}
+
+int pipeline_truncate(TileDesc *d_tiles, int ntiles, double *ws,
+        const double *d_bnorm, int kmax, double eta_tol, int nmax,
+        hipStream_t s)
+{
+    (void)nmax;
+    if(ntiles <= 0)
+        return STARSH_SUCCESS;
+    hipLaunchKernelGGL(truncate_kernel, dim3(ntiles), dim3(256), 0, s,
+            d_tiles, ws, d_bnorm, kmax, eta_tol);
+    HIP_TRY(hipGetLastError());
+    return STARSH_SUCCESS;
+}

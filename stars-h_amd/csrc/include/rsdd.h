@@ -1,0 +1,57 @@
+/* Internal C interface between the host drivers and the HIP pipeline of the
+ * randomized SVD (one batch of tiles). */
+#ifndef STARSH_RSDD_H
+#define STARSH_RSDD_H
+
+#include <stdint.h>
+#include "starsh-hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rsdd_input
+{
+    int ntiles;
+    const int *m, *n;            /* tile sizes */
+    /* tile contents: either generated ... */
+    const starsh_hip_kernel_desc *kd;
+    const double *d_xr, *d_xc;   /* points in cluster order (device, SoA) */
+    int64_t nr, nc;
+    const int64_t *r0, *c0;      /* first row / column point of each tile */
+    /* ... or explicit (column-major, ld = m) */
+    const double *d_a;
+    const int64_t *a_off;
+    int maxrank, oversample;
+    double tol;
+    double *S_out;               /* optional: k singular values per tile */
+    const int64_t *soff;
+    int *rows_out;               /* optional: sketch width k per tile */
+} RsddInput;
+
+typedef struct rsdd_output
+{
+    int *rank;                   /* ntiles: rank, -1 for a false far tile */
+    int *p, *kc, *flags;         /* ntiles: diagnostics */
+    int64_t *uoff, *voff;        /* ntiles: offsets of U, V in d_uv */
+    double *d_uv;                /* device buffer with U and V (caller frees
+                                    with starsh_hip_free) */
+    double device_seconds;
+    double sketch_seconds;
+} RsddOutput;
+
+int rsdd_batch(const RsddInput *in, RsddOutput *out);
+
+/* Per tile |A|_F^2 and |A - U V^T|_F^2 with A regenerated on the GPU. */
+int starsh_hip_tile_errors(const starsh_hip_kernel_desc *kd,
+        const double *d_xr, int64_t nr, const double *d_xc, int64_t nc,
+        int ntiles, const int64_t *r0, const int64_t *c0, const int *m,
+        const int *n, const int *rank, const double *d_U,
+        const int64_t *uoff, const double *d_V, const int64_t *voff,
+        double *norm2, double *diff2);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
