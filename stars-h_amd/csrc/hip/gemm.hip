@@ -221,8 +221,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(const TileDesc *tiles,
         }
         __syncthreads();
     }
-    // ---- epilogue: lane holds rows 4*(lane/16)+v, column lane%16 of each
-    // 16 x 16 block.
+    // ---- epilogue: lane holds rows lane/16 + 4*v, column lane%16 of each
+    // 16 x 16 block (v_mfma_f64_16x16x4f64 output layout).
 #pragma unroll
     for(int a = 0; a < 2; a++)
 #pragma unroll
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(const TileDesc *tiles,
 #pragma unroll
             for(int v = 0; v < 4; v++)
             {
-                const int gi = row0+wm*32+a*16+4*(lane >> 4)+v;
+                const int gi = row0+wm*32+a*16+(lane >> 4)+4*v;
                 if(gi >= o.M)
                     continue;
                 double *c = o.C+(int64_t)gj*o.ldc+gi;

@@ -68,6 +68,28 @@ int bcgs2(TileDesc *d_tiles, int ntiles, int which, double *ws,
     return STARSH_SUCCESS;
 }
 
+// Diagnostic: when STARSH_HIP_DEBUG_DUMP names a directory, the buffers of
+// tile 0 are written there after each stage (raw doubles, column-major).
+void dump(const char *name, const double *d_src, int64_t count, hipStream_t s)
+{
+    const char *dir = getenv("STARSH_HIP_DEBUG_DUMP");
+    if(dir == nullptr || count <= 0)
+        return;
+    std::vector<double> h(count);
+    if(hipStreamSynchronize(s) != hipSuccess ||
+            hipMemcpy(h.data(), d_src, sizeof(double)*count,
+                hipMemcpyDeviceToHost) != hipSuccess)
+        return;
+    char path[1024];
+    snprintf(path, sizeof path, "%s/%s.bin", dir, name);
+    FILE *f = fopen(path, "wb");
+    if(f)
+    {
+        fwrite(h.data(), sizeof(double), count, f);
+        fclose(f);
+    }
+}
+
 } // namespace
 
 // Runs the pipeline on ntiles tiles.  Either the kernel (kd, points) or
@@ -169,14 +191,19 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     TRY(pipeline_gemm(in->d_a ? GM_SKETCH_MEM : GM_SKETCH_GEN, d_tiles, nt,
                 mmax, kmax, g, s));
     HIP_TRY(hipEventRecord(ev[3], s));
+    dump("omega", d_ws+h[0].om_off, (int64_t)h[0].n*h[0].k, s);
+    dump("Y", d_ws+h[0].q_off, (int64_t)h[0].m*h[0].k, s);
     // 2. orthonormal basis with deflation, kept columns to the front
     TRY(pipeline_colnorm(d_tiles, nt, 0, d_ws, (double *)norms.p, kmax, s));
     TRY(bcgs2(d_tiles, nt, 0, d_ws, (double *)norms.p, kmax, kmax, mmax,
                 (int *)keep.p, g, s));
+    dump("Qfull", d_ws+h[0].q_off, (int64_t)h[0].m*h[0].k, s);
     TRY(pipeline_compact(d_tiles, nt, d_ws, (int *)keep.p, kmax, s));
+    dump("Q", d_ws+h[0].q_off, (int64_t)h[0].m*h[0].k, s);
     // 3. projection Bt = A^T Q
     TRY(pipeline_gemm(in->d_a ? GM_PROJ_MEM : GM_PROJ_GEN, d_tiles, nt, nmax,
                 kmax, g, s));
+    dump("Bt", d_ws+h[0].bt_off, (int64_t)h[0].n*h[0].k, s);
     // 4. truncation of B's rows
     TRY(pipeline_colnorm(d_tiles, nt, 2, d_ws, (double *)norms.p, kmax, s));
     TRY(pipeline_truncate(d_tiles, nt, d_ws, (double *)norms.p, kmax,
@@ -192,7 +219,9 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     {
         TRY(bcgs2(d_tiles, nt, 1, d_ws, (double *)norms.p, kmax, pmax, nmax,
                     (int *)keep.p, g, s));
+        dump("Q2", d_ws+h[0].q2_off, (int64_t)h[0].n*h[0].p, s);
         TRY(pipeline_gemm(GM_R2, d_tiles, nt, pmax, pmax, g, s));
+        dump("R2", d_ws+h[0].r_off, (int64_t)h[0].k*h[0].k, s);
     }
     // 6. core SVD and rank
     TRY(pipeline_core(d_tiles, nt, d_ws, in->maxrank, in->tol, pmax, s));
