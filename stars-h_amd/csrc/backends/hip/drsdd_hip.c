@@ -45,6 +45,8 @@ int starsh_hip_set_option(const char *name, double value)
         workspace_budget = (size_t)(value*(double)(1 << 30));
         return STARSH_SUCCESS;
     }
+    if(!strcmp(name, "mixed_first_bucket"))
+        return hip_mixed_set_first_bucket((int)value);
     return STARSH_WRONG_PARAMETER;
 }
 
@@ -388,12 +390,144 @@ int hip_driver_finish(STARSH_blrm **matrix, STARSH_blrf *format,
             alloc_U, alloc_V, alloc_D, '1');
 }
 
-/* Bytes of the plain pipeline's workspace for one tile. */
+/* Bytes of the pipeline's workspace for one tile. */
 static size_t tile_bytes(int m, int n, int k, int maxrank)
 {
     size_t r = (size_t)(maxrank < k ? maxrank : k);
     return sizeof(double)*((size_t)m*k+2*(size_t)n*k+4*(size_t)k*k+
-            17*(size_t)k+(size_t)(m+n)*r);
+            17*(size_t)k+(size_t)(m+n)*r+(size_t)m)+(size_t)n*k;
+}
+
+int hip_driver_run(const hip_problem *hp, STARSH_blrf *F, int maxrank,
+        double tol, STARSH_int ntiles, const STARSH_int *idx, const int *k,
+        int int8, double tau, hip_factors *hf, int *rank_out, int *flags_out,
+        int *accepted, hip_accept_fn accept, starsh_hip_stats *st)
+{
+    STARSH_cluster *RC = F->row_cluster, *CC = F->col_cluster;
+    if(ntiles == 0)
+        return STARSH_SUCCESS;
+    int info = STARSH_SUCCESS;
+    int *m = malloc(sizeof(int)*ntiles), *n = malloc(sizeof(int)*ntiles);
+    int *pp = malloc(sizeof(int)*ntiles), *kc = malloc(sizeof(int)*ntiles);
+    int64_t *r0 = malloc(sizeof(int64_t)*ntiles);
+    int64_t *c0 = malloc(sizeof(int64_t)*ntiles);
+    int64_t *uo = malloc(sizeof(int64_t)*ntiles);
+    int64_t *vo = malloc(sizeof(int64_t)*ntiles);
+    if(!m || !n || !pp || !kc || !r0 || !c0 || !uo || !vo)
+        info = STARSH_MALLOC_ERROR;
+    for(STARSH_int t = 0; info == STARSH_SUCCESS && t < ntiles; t++)
+    {
+        STARSH_int bi = idx[t];
+        STARSH_int i = F->block_far[2*bi], j = F->block_far[2*bi+1];
+        m[t] = (int)RC->size[i];
+        n[t] = (int)CC->size[j];
+        r0[t] = RC->start[i];
+        c0[t] = CC->start[j];
+        if(m[t] != n[t])
+            STARSH_WARNING("This was only tested on square tiles, error of "
+                    "approximation may be much higher, than demanded");
+    }
+    STARSH_int b0 = 0;
+    while(info == STARSH_SUCCESS && b0 < ntiles)
+    {
+        size_t bytes = 0;
+        STARSH_int b1 = b0;
+        while(b1 < ntiles)
+        {
+            int mn = m[b1] < n[b1] ? m[b1] : n[b1];
+            int kk = k[b1] < mn ? k[b1] : mn;
+            size_t tb = tile_bytes(m[b1], n[b1], kk, maxrank);
+            if(b1 > b0 && bytes+tb > hip_driver_workspace_budget())
+                break;
+            bytes += tb;
+            st->flops_algorithmic += 4.0*m[b1]*(double)n[b1]*
+                (maxrank+hp->oversample < mn ? maxrank+hp->oversample : mn);
+            st->sketch_bytes_or_flops += 2.0*m[b1]*(double)n[b1]*kk;
+            b1++;
+        }
+        RsddInput in;
+        memset(&in, 0, sizeof(in));
+        in.ntiles = (int)(b1-b0);
+        in.m = m+b0;
+        in.n = n+b0;
+        in.k = k+b0;
+        in.kd = &hp->kd;
+        in.d_xr = hp->d_xr;
+        in.d_xc = hp->d_xc;
+        in.nr = hp->nr;
+        in.nc = hp->nc;
+        in.r0 = r0+b0;
+        in.c0 = c0+b0;
+        in.maxrank = maxrank;
+        in.oversample = hp->oversample;
+        in.tol = tol;
+        in.sketch_int8 = int8;
+        in.tau = tau;
+        RsddOutput out;
+        memset(&out, 0, sizeof(out));
+        out.rank = rank_out+b0;
+        out.p = pp+b0;
+        out.kc = kc+b0;
+        out.flags = flags_out+b0;
+        out.uoff = uo+b0;
+        out.voff = vo+b0;
+        info = rsdd_batch(&in, &out);
+        if(info != STARSH_SUCCESS)
+            break;
+        st->device_seconds += out.device_seconds;
+        st->dominant_kernel_seconds += out.sketch_seconds;
+        st->dominant_kernel_launches += 1;
+        double tc = hip_driver_now();
+        for(STARSH_int t = b0; t < b1 && info == STARSH_SUCCESS; t++)
+        {
+            int kk = k[t] < (m[t] < n[t] ? m[t] : n[t]) ? k[t] :
+                (m[t] < n[t] ? m[t] : n[t]);
+            int kfull = maxrank+hp->oversample;
+            int mn = m[t] < n[t] ? m[t] : n[t];
+            kfull = kfull < mn ? kfull : mn;
+            accepted[t] = accept ? accept(rank_out[t], kk, kfull,
+                    flags_out[t], hp->oversample) : 1;
+            if(!accepted[t])
+                continue;
+            STARSH_int bi = idx[t];
+            hf->far_rank[bi] = rank_out[t];
+            if(rank_out[t] <= 0)
+                continue;
+            info = starsh_hip_memcpy_d2h(hf->alloc_U+hf->uoff[bi],
+                    out.d_uv+uo[t], sizeof(double)*(size_t)m[t]*rank_out[t]);
+            if(info == STARSH_SUCCESS)
+                info = starsh_hip_memcpy_d2h(hf->alloc_V+hf->voff[bi],
+                        out.d_uv+vo[t], sizeof(double)*(size_t)n[t]*
+                        rank_out[t]);
+        }
+        st->copy_seconds += hip_driver_now()-tc;
+        starsh_hip_free(out.d_uv);
+        b0 = b1;
+    }
+    free(m); free(n); free(pp); free(kc); free(r0); free(c0); free(uo);
+    free(vo);
+    return info;
+}
+
+void hip_driver_rank_stats(const hip_factors *hf, STARSH_int nfar,
+        starsh_hip_stats *st)
+{
+    long long cnt = 0, sum = 0;
+    st->rank_min = 1 << 30;
+    st->rank_max = -1;
+    for(STARSH_int bi = 0; bi < nfar; bi++)
+        if(hf->far_rank[bi] >= 0)
+        {
+            int r = hf->far_rank[bi];
+            cnt++;
+            sum += r;
+            st->rank_min = r < st->rank_min ? r : st->rank_min;
+            st->rank_max = r > st->rank_max ? r : st->rank_max;
+        }
+    st->rank_avg = cnt ? (double)sum/cnt : 0;
+    if(cnt == 0)
+        st->rank_min = 0;
+    st->ntiles = nfar;
 }
 
 int starsh_blrm__drsdd_hip(STARSH_blrm **matrix, STARSH_blrf *format,
@@ -410,125 +544,28 @@ int starsh_blrm__drsdd_hip(STARSH_blrm **matrix, STARSH_blrf *format,
     if(info != STARSH_SUCCESS)
         return info;
     STARSH_blrf *F = format;
-    STARSH_cluster *RC = F->row_cluster, *CC = F->col_cluster;
     STARSH_int nfar = F->nblocks_far;
     hip_factors hf;
     info = hip_driver_alloc_factors(F, maxrank, &hf);
     starsh_hip_stats st;
     memset(&st, 0, sizeof(st));
-    st.ntiles = nfar;
-    int *m = NULL, *n = NULL, *rk = NULL, *pp = NULL, *kc = NULL, *fl = NULL;
-    int64_t *r0 = NULL, *c0 = NULL, *uo = NULL, *vo = NULL;
-    if(info == STARSH_SUCCESS && nfar > 0)
-    {
-        m = malloc(sizeof(int)*nfar);
-        n = malloc(sizeof(int)*nfar);
-        rk = malloc(sizeof(int)*nfar);
-        pp = malloc(sizeof(int)*nfar);
-        kc = malloc(sizeof(int)*nfar);
-        fl = malloc(sizeof(int)*nfar);
-        r0 = malloc(sizeof(int64_t)*nfar);
-        c0 = malloc(sizeof(int64_t)*nfar);
-        uo = malloc(sizeof(int64_t)*nfar);
-        vo = malloc(sizeof(int64_t)*nfar);
-        if(!m || !n || !rk || !pp || !kc || !fl || !r0 || !c0 || !uo || !vo)
-            info = STARSH_MALLOC_ERROR;
-    }
+    STARSH_int *idx = malloc(sizeof(STARSH_int)*(nfar+1));
+    int *k = malloc(sizeof(int)*(nfar+1)), *rk = malloc(sizeof(int)*(nfar+1));
+    int *fl = malloc(sizeof(int)*(nfar+1)), *acc = malloc(sizeof(int)*
+            (nfar+1));
+    if(info == STARSH_SUCCESS && (!idx || !k || !rk || !fl || !acc))
+        info = STARSH_MALLOC_ERROR;
     for(STARSH_int bi = 0; info == STARSH_SUCCESS && bi < nfar; bi++)
     {
-        STARSH_int i = F->block_far[2*bi], j = F->block_far[2*bi+1];
-        m[bi] = (int)RC->size[i];
-        n[bi] = (int)CC->size[j];
-        r0[bi] = RC->start[i];
-        c0[bi] = CC->start[j];
-        if(m[bi] != n[bi])
-            STARSH_WARNING("This was only tested on square tiles, error of "
-                    "approximation may be much higher, than demanded");
-    }
-    /* Chunks of tiles whose workspace fits the budget. */
-    STARSH_int b0 = 0;
-    double flops = 0;
-    while(info == STARSH_SUCCESS && b0 < nfar)
-    {
-        size_t bytes = 0;
-        STARSH_int b1 = b0;
-        while(b1 < nfar)
-        {
-            int mn = m[b1] < n[b1] ? m[b1] : n[b1];
-            int k = maxrank+hp.oversample < mn ? maxrank+hp.oversample : mn;
-            size_t tb = tile_bytes(m[b1], n[b1], k, maxrank);
-            if(b1 > b0 && bytes+tb > hip_driver_workspace_budget())
-                break;
-            bytes += tb;
-            flops += 4.0*m[b1]*(double)n[b1]*k;
-            b1++;
-        }
-        RsddInput in;
-        memset(&in, 0, sizeof(in));
-        in.ntiles = (int)(b1-b0);
-        in.m = m+b0;
-        in.n = n+b0;
-        in.kd = &hp.kd;
-        in.d_xr = hp.d_xr;
-        in.d_xc = hp.d_xc;
-        in.nr = hp.nr;
-        in.nc = hp.nc;
-        in.r0 = r0+b0;
-        in.c0 = c0+b0;
-        in.maxrank = maxrank;
-        in.oversample = hp.oversample;
-        in.tol = tol;
-        RsddOutput out;
-        memset(&out, 0, sizeof(out));
-        out.rank = rk+b0;
-        out.p = pp+b0;
-        out.kc = kc+b0;
-        out.flags = fl+b0;
-        out.uoff = uo+b0;
-        out.voff = vo+b0;
-        info = rsdd_batch(&in, &out);
-        if(info != STARSH_SUCCESS)
-            break;
-        st.device_seconds += out.device_seconds;
-        st.dominant_kernel_seconds += out.sketch_seconds;
-        st.dominant_kernel_launches += 1;
-        double tc = hip_driver_now();
-        for(STARSH_int bi = b0; bi < b1 && info == STARSH_SUCCESS; bi++)
-        {
-            hf.far_rank[bi] = rk[bi];
-            if(rk[bi] <= 0)
-                continue;
-            info = starsh_hip_memcpy_d2h(hf.alloc_U+hf.uoff[bi],
-                    out.d_uv+uo[bi], sizeof(double)*(size_t)m[bi]*rk[bi]);
-            if(info == STARSH_SUCCESS)
-                info = starsh_hip_memcpy_d2h(hf.alloc_V+hf.voff[bi],
-                        out.d_uv+vo[bi], sizeof(double)*(size_t)n[bi]*rk[bi]);
-        }
-        st.copy_seconds += hip_driver_now()-tc;
-        starsh_hip_free(out.d_uv);
-        b0 = b1;
+        idx[bi] = bi;
+        k[bi] = maxrank+hp.oversample;
     }
     if(info == STARSH_SUCCESS)
-    {
-        long long cnt = 0, sum = 0;
-        st.rank_min = 1 << 30;
-        st.rank_max = -1;
-        for(STARSH_int bi = 0; bi < nfar; bi++)
-            if(rk[bi] >= 0)
-            {
-                cnt++;
-                sum += rk[bi];
-                st.rank_min = rk[bi] < st.rank_min ? rk[bi] : st.rank_min;
-                st.rank_max = rk[bi] > st.rank_max ? rk[bi] : st.rank_max;
-            }
-        st.rank_avg = cnt ? (double)sum/cnt : 0;
-        if(cnt == 0)
-            st.rank_min = 0;
-        st.sketch_bytes_or_flops = flops/2;
-        st.flops_algorithmic = flops;
-    }
-    free(m); free(n); free(rk); free(pp); free(kc); free(fl);
-    free(r0); free(c0); free(uo); free(vo);
+        info = hip_driver_run(&hp, F, maxrank, tol, nfar, idx, k, 0, 0.0,
+                &hf, rk, fl, acc, NULL, &st);
+    if(info == STARSH_SUCCESS)
+        hip_driver_rank_stats(&hf, nfar, &st);
+    free(idx); free(k); free(rk); free(fl); free(acc);
     if(info == STARSH_SUCCESS)
         info = hip_driver_finish(matrix, F, &hp, &hf, onfly, &st);
     if(info != STARSH_SUCCESS)

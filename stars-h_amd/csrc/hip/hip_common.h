@@ -32,7 +32,9 @@ struct TileDesc
     int64_t coef_off;  // projection coefficients (k x 32, ld k)
     int64_t u_off;     // U (m x maxrank, ld m)
     int64_t v_off;     // V (n x maxrank, ld n)
-    int kc;            // kept (numerically independent) sketch columns
+    int64_t rs_off;    // row scales of the int8 sketch (m)
+    int64_t om8_off;   // integer test matrix (bytes, n x k, ld n)
+    int kc;           // kept (numerically independent) sketch columns
     int p;             // size of the core problem after truncation
     int rank;          // chosen rank, -1 for a false far-field tile
     int flags;

@@ -53,6 +53,14 @@ int pipeline_truncate(TileDesc *d_tiles, int ntiles, double *ws,
         const double *d_bnorm, int kmax, double eta_tol, int nmax,
         hipStream_t s);
 
+// Integer test matrix (n x k bytes) for the int8 sketch.
+int pipeline_omega_i8(signed char *d_out, int ncols, int k, hipStream_t s);
+
+// Row scales and Y = A Omega with int8 digit planes (mixed form).
+int pipeline_sketch_i8(const TileDesc *d_tiles, int ntiles, int mmax,
+        int kmax, const GemmArgs &g, const signed char *d_om8,
+        hipStream_t s);
+
 // One-sided Jacobi SVD of R2, singular values, rank by the reference rule,
 // W_r and J_r.
 int pipeline_core(TileDesc *d_tiles, int ntiles, double *ws, int maxrank,

@@ -43,7 +43,7 @@ struct DeviceBuf
 
 int bcgs2(TileDesc *d_tiles, int ntiles, int which, double *ws,
         const double *d_ref, int kmax, int width_max, int rows_max,
-        int *d_keep, GemmArgs g, hipStream_t s)
+        int *d_keep, double tau_first, GemmArgs g, hipStream_t s)
 {
     const int pb = rows_max <= 2048 ? 16 : 8;
     g.which = which;
@@ -61,7 +61,7 @@ int bcgs2(TileDesc *d_tiles, int ntiles, int which, double *ws,
             }
             TRY(pipeline_panel(d_tiles, ntiles, which, ws, j0, pb,
                         pass == 0 ? d_ref : nullptr, kmax,
-                        pass == 0 ? TAU_FIRST : TAU_SECOND, d_keep, rows_max,
+                        pass == 0 ? tau_first : TAU_SECOND, d_keep, rows_max,
                         s));
         }
     }
@@ -115,9 +115,11 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
         d.m = in->m[t];
         d.n = in->n[t];
         int mn = d.m < d.n ? d.m : d.n;
-        d.k = in->maxrank+in->oversample;
+        d.k = in->k ? in->k[t] : in->maxrank+in->oversample;
         if(d.k > mn)
             d.k = mn;
+        if(d.k < 1)
+            d.k = 1;
         d.gen = in->d_a == nullptr;
         d.r0 = in->r0 ? in->r0[t] : 0;
         d.c0 = in->c0 ? in->c0[t] : 0;
@@ -126,16 +128,29 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
         mmax = d.m > mmax ? d.m : mmax;
         nmax = d.n > nmax ? d.n : nmax;
     }
-    // Workspace layout: all Omega first, then per-tile regions.
+    // Workspace layout: all Omega first, then per-tile regions.  The int8
+    // sketch keeps its integer Omega in a separate byte buffer.
+    int64_t off8 = 0;
     for(int t = 0; t < nt; t++)
     {
         auto key = std::make_pair(h[t].n, h[t].k);
         if(omegas.find(key) == omegas.end())
         {
-            omegas[key] = off;
-            off += (int64_t)h[t].n*h[t].k;
+            if(in->sketch_int8)
+            {
+                omegas[key] = off8;
+                off8 += ((int64_t)h[t].n*h[t].k+15)/16*16;
+            }
+            else
+            {
+                omegas[key] = off;
+                off += (int64_t)h[t].n*h[t].k;
+            }
         }
-        h[t].om_off = omegas[key];
+        if(in->sketch_int8)
+            h[t].om8_off = omegas[key];
+        else
+            h[t].om_off = omegas[key];
     }
     int64_t uvoff = 0;
     for(int t = 0; t < nt; t++)
@@ -150,6 +165,7 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
         d.w_off = off; off += 2*k*k;
         d.s_off = off; off += k;
         d.coef_off = off; off += k*16;
+        d.rs_off = off; off += in->sketch_int8 ? m : 0;
         d.u_off = uvoff; uvoff += m*rmax;
         d.v_off = uvoff; uvoff += n*rmax;
         out->uoff[t] = d.u_off;
@@ -158,8 +174,10 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     if(in->rows_out)
         for(int t = 0; t < nt; t++)
             in->rows_out[t] = h[t].k;
-    DeviceBuf ws, tiles, keep, norms, uv;
+    DeviceBuf ws, tiles, keep, norms, uv, om8;
     TRY(ws.alloc(sizeof(double)*off));
+    if(in->sketch_int8)
+        TRY(om8.alloc(off8));
     TRY(tiles.alloc(sizeof(TileDesc)*nt));
     TRY(keep.alloc(sizeof(int)*(size_t)nt*kmax));
     TRY(norms.alloc(sizeof(double)*(size_t)nt*kmax));
@@ -169,7 +187,14 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     HIP_TRY(hipMemcpyAsync(d_tiles, h.data(), sizeof(TileDesc)*nt,
                 hipMemcpyHostToDevice, s));
     for(auto &o : omegas)
-        TRY(starsh_hip_omega(d_ws+o.second, o.first.first, o.first.second));
+    {
+        if(in->sketch_int8)
+            TRY(pipeline_omega_i8((signed char *)om8.p+o.second,
+                        o.first.first, o.first.second, s));
+        else
+            TRY(starsh_hip_omega(d_ws+o.second, o.first.first,
+                        o.first.second));
+    }
 
     GemmArgs g;
     memset(&g, 0, sizeof(g));
@@ -188,15 +213,23 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     HIP_TRY(hipEventRecord(ev[0], s));
     // 1. sketch
     HIP_TRY(hipEventRecord(ev[2], s));
-    TRY(pipeline_gemm(in->d_a ? GM_SKETCH_MEM : GM_SKETCH_GEN, d_tiles, nt,
-                mmax, kmax, g, s));
+    if(in->sketch_int8)
+    {
+        if(in->d_a != nullptr || in->kd == nullptr)
+            return STARSH_WRONG_PARAMETER;
+        TRY(pipeline_sketch_i8(d_tiles, nt, mmax, kmax, g,
+                    (const signed char *)om8.p, s));
+    }
+    else
+        TRY(pipeline_gemm(in->d_a ? GM_SKETCH_MEM : GM_SKETCH_GEN, d_tiles,
+                    nt, mmax, kmax, g, s));
     HIP_TRY(hipEventRecord(ev[3], s));
     dump("omega", d_ws+h[0].om_off, (int64_t)h[0].n*h[0].k, s);
     dump("Y", d_ws+h[0].q_off, (int64_t)h[0].m*h[0].k, s);
     // 2. orthonormal basis with deflation, kept columns to the front
     TRY(pipeline_colnorm(d_tiles, nt, 0, d_ws, (double *)norms.p, kmax, s));
     TRY(bcgs2(d_tiles, nt, 0, d_ws, (double *)norms.p, kmax, kmax, mmax,
-                (int *)keep.p, g, s));
+                (int *)keep.p, in->tau > 0 ? in->tau : TAU_FIRST, g, s));
     dump("Qfull", d_ws+h[0].q_off, (int64_t)h[0].m*h[0].k, s);
     TRY(pipeline_compact(d_tiles, nt, d_ws, (int *)keep.p, kmax, s));
     dump("Q", d_ws+h[0].q_off, (int64_t)h[0].m*h[0].k, s);
@@ -218,7 +251,7 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     if(pmax > 0)
     {
         TRY(bcgs2(d_tiles, nt, 1, d_ws, (double *)norms.p, kmax, pmax, nmax,
-                    (int *)keep.p, g, s));
+                    (int *)keep.p, TAU_FIRST, g, s));
         dump("Q2", d_ws+h[0].q2_off, (int64_t)h[0].n*h[0].p, s);
         TRY(pipeline_gemm(GM_R2, d_tiles, nt, pmax, pmax, g, s));
         dump("R2", d_ws+h[0].r_off, (int64_t)h[0].k*h[0].k, s);

@@ -30,6 +30,24 @@ int hip_driver_finish(STARSH_blrm **matrix, STARSH_blrf *format,
         const hip_problem *hp, hip_factors *hf, int onfly,
         starsh_hip_stats *st);
 void hip_driver_set_stats(const starsh_hip_stats *stats);
+
+/* Decides whether a tile's result is final: (rank, sketch width used, full
+ * sketch width of the plain form, pipeline flags, oversample). */
+typedef int (*hip_accept_fn)(int rank, int k, int kfull, int flags,
+        int oversample);
+
+/* Compresses the far-field tiles idx[0..ntiles) with sketch widths k[] in
+ * workspace-sized chunks; accepted results (all when accept is NULL) go to
+ * hf (rank and U/V copied to the host buffers). */
+int hip_driver_run(const hip_problem *hp, STARSH_blrf *F, int maxrank,
+        double tol, STARSH_int ntiles, const STARSH_int *idx, const int *k,
+        int int8, double tau, hip_factors *hf, int *rank_out, int *flags_out,
+        int *accepted, hip_accept_fn accept, starsh_hip_stats *st);
+void hip_driver_rank_stats(const hip_factors *hf, STARSH_int nfar,
+        starsh_hip_stats *st);
+
+/* First rank bucket (index into 32, 64, 128, 256, 512) of the mixed form. */
+int hip_mixed_set_first_bucket(int b);
 size_t hip_driver_workspace_budget(void);
 double hip_driver_now(void);
 

@@ -24,6 +24,11 @@ typedef struct rsdd_input
     const int64_t *a_off;
     int maxrank, oversample;
     double tol;
+    const int *k;                /* optional per-tile sketch width (default
+                                    min(maxrank+oversample, m, n)) */
+    int sketch_int8;             /* 1: int8-digit sketch (mixed form) */
+    double tau;                  /* deflation threshold of the sketch
+                                    columns (0: 1e-13) */
     double *S_out;               /* optional: k singular values per tile */
     const int64_t *soff;
     int *rows_out;               /* optional: sketch width k per tile */

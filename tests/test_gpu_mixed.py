@@ -1,0 +1,105 @@
+"""GPU checks of the batched mixed-precision form (STARSH_BACKEND=HIP_MIXED).
+
+The mixed form does not promise the reference's exact ranks (its sketch
+width adapts per tile); it promises the requested accuracy.  Tests check the
+int8 sketch against an exact product, per-tile and global relative
+Frobenius errors against FP64 regenerated tiles (starsh_blrm__dfe_hip and
+the oracle), and ranks against the plain FP64 form.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import starsh_hip as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def test_int8_sketch_matches_exact_product(tmp_path):
+    """Y from the int8 digit planes equals A @ round(32*Omega) / 32 up to
+    the 2^-28 representation error of each row of A."""
+    _say("case: int8 sketch of 2D sqexp N=3200 nb=800")
+    os.environ["STARSH_HIP_DEBUG_DUMP"] = str(tmp_path)
+    L = S.lib()
+    # one bucket only (the widest), so the dump holds tile (0, 0)
+    assert L.starsh_hip_set_option(b"mixed_first_bucket", C.c_double(4)) == 0
+    try:
+        fn = S.init("HIP_MIXED", oversample=16)
+        p = S.TLRProblem("spatial", 3200, 2, 800, symm="S", seed=4,
+                         kernel="sqrexp", beta=0.1)
+        assert p.approximate(200, 1e-7, 0, fn) == 0
+    finally:
+        del os.environ["STARSH_HIP_DEBUG_DUMP"]
+        L.starsh_hip_set_option(b"mixed_first_bucket", C.c_double(0))
+    Y = np.fromfile(tmp_path / "Y.bin")
+    m = 800
+    k = Y.size // m
+    Y = Y.reshape(k, m).T
+    pts = p.points()
+    st, sz = p.cluster_ranges()
+    i, j = 0, 0
+    A = O.kernel_block(O.kernel("sqrexp", 2, beta=0.1), pts, pts,
+                       np.arange(st[i], st[i] + sz[i]),
+                       np.arange(st[j], st[j] + sz[j]))
+    om = np.clip(np.rint(32.0 * O.omega(800, k)), -127, 127)
+    want = A @ om / 32.0
+    rowmax = np.max(np.abs(A), axis=1)
+    bound = 2.0 ** -27 * rowmax * np.sum(np.abs(om), axis=0).max() / 32.0
+    assert np.all(np.max(np.abs(Y - want), axis=1) <= bound + 1e-300)
+    p.free_matrix()
+
+
+def _mixed_vs_plain(kind, N, ndim, nb, maxrank, os_, tol, **kw):
+    res = {}
+    for backend in ("HIP", "HIP_MIXED"):
+        fn = S.init(backend, oversample=os_)
+        p = S.TLRProblem(kind, N, ndim, nb, symm="S", **kw)
+        assert p.approximate(maxrank, tol, 0, fn) == 0
+        g, terr, tnorm = p.dfe_tiles()
+        res[backend] = (p.ranks(), g, terr, tnorm, p.tiles(), p)
+    return res
+
+
+def test_mixed_sqexp2d_accuracy_and_ranks():
+    _say("case: mixed vs plain, 2D sqexp N=12800 nb=1600 tol 1e-7")
+    res = _mixed_vs_plain("spatial", 12800, 2, 1600, 400, 16, 1e-7, seed=1,
+                          kernel="sqrexp", beta=0.1)
+    rp, gp, _, _, tp, pp = res["HIP"]
+    rm, gm, terr, tnorm, tm, pm = res["HIP_MIXED"]
+    _say(f"  plain err {gp:.3e} mixed err {gm:.3e}; ranks plain "
+         f"{rp.min()}..{rp.max()} mixed {rm.min()}..{rm.max()}")
+    assert tp == tm
+    assert gm <= 1e-7
+    big = tnorm > 1e-12 * tnorm.max()
+    assert np.all(terr[big] <= 2e-7)
+    assert np.max(np.abs(rm - rp)) <= 3
+    # oracle check of the mixed factors' error
+    pts = pm.points()
+    st, sz = pm.cluster_ranges()
+    Us = [pm.factors(b)[0] for b in range(len(rm))]
+    Vs = [pm.factors(b)[1] for b in range(len(rm))]
+    go, _, _ = O.dfe(O.kernel("sqrexp", 2, beta=0.1), pts, st, sz, "S", tm,
+                     rm, Us, Vs, pm.near_tiles())
+    assert abs(go - gm) <= 1e-6 * go
+    pp.free_matrix()
+    pm.free_matrix()
+
+
+def test_mixed_exp2d_tight_tolerance_uses_fp64_sketch():
+    _say("case: mixed vs plain, 2D exp N=2500 nb=250 tol 1e-9")
+    res = _mixed_vs_plain("spatial", 2500, 2, 250, 100, 10, 1e-9, seed=0,
+                          kernel="exp", beta=0.1)
+    rp, gp, _, _, tp, pp = res["HIP"]
+    rm, gm, _, _, tm, pm = res["HIP_MIXED"]
+    _say(f"  plain err {gp:.3e} mixed err {gm:.3e}")
+    assert gm / 1e-9 <= 10
+    assert len(pm.near_tiles()) == len(pp.near_tiles())
+    pp.free_matrix()
+    pm.free_matrix()
