@@ -32,13 +32,17 @@ int hip_mixed_set_first_bucket(int b)
     return STARSH_SUCCESS;
 }
 
+/* A randomized range finder with k = r + p samples leaves an expected
+ * error factor sqrt(1 + r/(p-1)) over the best rank-r approximation; the
+ * reference keeps p = maxrank + oversample - r.  A bucket result is final
+ * when p >= max(oversample, r), i.e. the factor stays below sqrt(2). */
 static int accept_mixed(int rank, int k, int kfull, int flags, int os)
 {
     if(k >= kfull)
         return 1;
     if(rank < 0)
         return 0;
-    if(rank+os <= k)
+    if(rank+(rank > os ? rank : os) <= k)
         return 1;
     /* flag 1: a whole panel of sketch columns was numerically dependent */
     return (flags & 1) != 0;

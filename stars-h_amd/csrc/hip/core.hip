@@ -111,6 +111,9 @@ __global__ __launch_bounds__(CORE_THREADS) void core_kernel(TileDesc *tiles,
     const int p = td.p, k = td.k;
     if(p <= 0)
     {
+        // zero sketch: all singular values vanish and the rank is 0
+        for(int j = threadIdx.x; j < k; j += blockDim.x)
+            ws[td.s_off+j] = 0.0;
         if(threadIdx.x == 0)
             td.rank = 0;
         return;

@@ -99,7 +99,7 @@ def test_mixed_exp2d_tight_tolerance_uses_fp64_sketch():
     rp, gp, _, _, tp, pp = res["HIP"]
     rm, gm, _, _, tm, pm = res["HIP_MIXED"]
     _say(f"  plain err {gp:.3e} mixed err {gm:.3e}")
-    assert gm / 1e-9 <= 10
-    assert len(pm.near_tiles()) == len(pp.near_tiles())
+    assert gm <= max(2 * gp, 1e-9)
+    assert pm.near_tiles() == pp.near_tiles()
     pp.free_matrix()
     pm.free_matrix()
