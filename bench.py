@@ -26,10 +26,16 @@ N, NB, MAXRANK, OVERSAMPLE, TOL, BETA, SEED = 64000, 1600, 400, 16, 1e-7, \
 PEAK_FP64 = 78.6  # TFLOP/s, MI355X dense FP64 (vector and matrix)
 
 
-def run_form(S, backend, steps, warmup):
+def run_form(S, backend, steps, warmup, autotune):
     fn = S.init(backend, oversample=OVERSAMPLE)
     p = S.TLRProblem("spatial", N, 2, NB, symm="S", seed=SEED,
                      kernel="sqrexp", beta=BETA)
+    schedule = None
+    if backend == "HIP_MIXED" and autotune:
+        # plan step (outside the timed steps): first rank bucket by trial
+        first, secs = p.autotune(MAXRANK, TOL)
+        schedule = {"first_bucket_width": [32, 64, 128, 256, 512][first],
+                    "candidate_device_seconds": [round(x, 6) for x in secs]}
     dev, wall, dom, launches = [], [], 0.0, 0
     for it in range(warmup + steps):
         info = p.approximate(MAXRANK, TOL, 0, fn)
@@ -60,6 +66,8 @@ def run_form(S, backend, steps, warmup):
         "cluster": p.cluster_ranges(),
         "tiles_list": p.tiles(),
     }
+    if schedule is not None:
+        res["schedule"] = schedule
     p.free_matrix()
     return res
 
@@ -90,6 +98,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="plain,mixed")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the compression path runs on a "
@@ -103,7 +112,8 @@ def main():
         try:
             print(f"bench: {form} form, {args.warmup} warmup + {args.steps} "
                   f"steps", file=sys.stderr, flush=True)
-            forms[form] = run_form(S, backend, args.steps, args.warmup)
+            forms[form] = run_form(S, backend, args.steps, args.warmup,
+                                   not args.no_autotune)
         except AssertionError as e:
             print(f"bench: {form} form unavailable ({e})", file=sys.stderr)
     plain = forms.get("plain")
@@ -114,8 +124,9 @@ def main():
         print("bench: CPU baseline sample", file=sys.stderr, flush=True)
         cpu = cpu_baseline(plain["points"], plain["cluster"],
                            plain["tiles_list"], args.cpu_sample)
-    dom_t = plain["dominant_kernel_seconds"]
-    achieved = plain["sketch_flops"] / dom_t / 1e12 if dom_t > 0 else 0.0
+    dom = plain or head
+    dom_t = dom["dominant_kernel_seconds"]
+    achieved = dom["sketch_flops"] / dom_t / 1e12 if dom_t > 0 else 0.0
     summary = {}
     for name, r in forms.items():
         summary[name] = {k: (float(f"{v:.6g}") if isinstance(v, float)
@@ -143,7 +154,8 @@ def main():
         "forms": summary,
         "speedup_mixed_vs_plain": (round(plain["device_seconds"] /
                                          forms["mixed"]["device_seconds"], 3)
-                                   if "mixed" in forms else None),
+                                   if "mixed" in forms and plain
+                                   else None),
         "roofline": {"kernel": "sketch GEMM Y=A*Omega (FP64 MFMA, fused "
                                "tile generation), plain form",
                      "bound": "mfma", "achieved": round(achieved, 3),

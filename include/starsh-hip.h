@@ -47,6 +47,19 @@ int starsh_blrm__drsdd_hip(struct starsh_blrm **matrix,
 int starsh_blrm__drsdd_hip_mixed(struct starsh_blrm **matrix,
         struct starsh_blrf *format, int maxrank, double tol, int onfly);
 
+/* Batch of independent problems in one call (no reference counterpart: the
+ * reference compresses one problem per starsh_blrm_approximate call,
+ * include/starsh.h:563-567).  Equivalent to
+ *   for q < nprob: starsh_blrm_approximate(&matrix[q], format[q],
+ *                                          maxrank[q], tol[q], onfly)
+ * with the engine selected by starsh_init(); under the HIP backends the
+ * far-field tiles of all problems go through the GPU pipeline together
+ * (problems are grouped by kernel family and dimension).  On failure no
+ * matrix is returned (all matrix[q] are NULL or freed). */
+int starsh_blrm_approximate_batch(struct starsh_blrm **matrix,
+        struct starsh_blrf **format, int nprob, const int *maxrank,
+        const double *tol, int onfly);
+
 /* Global relative Frobenius error on the GPU; replaces starsh_blrm__dfe_omp
  * (src/backends/openmp/blrm/dfe.c:16-148).  Returns -1 on failure. */
 double starsh_blrm__dfe_hip(struct starsh_blrm *matrix);
@@ -78,10 +91,28 @@ typedef struct starsh_hip_stats
 
 int starsh_hip_get_stats(starsh_hip_stats *stats);
 
-/* Set to 1 to keep the last compression's factors resident on the device
- * after the copy-back, so that a following starsh_blrm__dfe_hip skips the
- * upload (no reference counterpart). */
+/* Tuning options of the HIP drivers (no reference counterpart):
+ *   "workspace_gb"        device workspace budget per batch (default 96);
+ *   "mixed_first_bucket"  first rank bucket of the mixed form, an index into
+ *                         {32, 64, 128, 256, 512}, or -1 (default) for the
+ *                         autotuner's schedule table (0 without an entry);
+ *   "gemm_bn"             column block of the generated-tile GEMMs: 0 (by
+ *                         size, default), 64, 128 or 256.
+ * Returns STARSH_WRONG_PARAMETER for an unknown name or value. */
 int starsh_hip_set_option(const char *name, double value);
+
+/* Autotuner of the mixed form (no reference counterpart): compresses
+ * *format once per candidate first rank bucket, keeps the one with the
+ * shortest device time and records it for this problem signature (kernel
+ * and its parameters, point count, tile size, maxrank, oversample, tol) in
+ * the in-process schedule table, which starsh_blrm__drsdd_hip_mixed
+ * consults, and appends it to the file named by STARSH_HIP_SCHEDULE when
+ * that variable is set (the file is read on first use).  *first_out gets
+ * the chosen bucket; seconds_out (5 entries, may be NULL) the device time
+ * of each candidate tried.  *format may be rewritten in place as by the
+ * drivers (false far-field tiles become near-field). */
+int starsh_hip_autotune(struct starsh_blrf *format, int maxrank, double tol,
+        int onfly, int *first_out, double *seconds_out);
 
 /* ---- Thin C ABI over the HIP kernels.  Plain pointers and sizes; device
  * pointers are marked d_.  Every function returns STARSH_SUCCESS or a

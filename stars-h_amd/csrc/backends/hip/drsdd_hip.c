@@ -47,6 +47,8 @@ int starsh_hip_set_option(const char *name, double value)
     }
     if(!strcmp(name, "mixed_first_bucket"))
         return hip_mixed_set_first_bucket((int)value);
+    if(!strcmp(name, "gemm_bn"))
+        return starsh_hip_gen_gemm_bn((int)value);
     return STARSH_WRONG_PARAMETER;
 }
 
@@ -242,7 +244,7 @@ int hip_driver_finish(STARSH_blrm **matrix, STARSH_blrf *format,
     for(bi = 0; bi < nblocks_far; bi++)
         if(far_rank[bi] == -1)
             nfalse++;
-    st->nfalse_far = nfalse;
+    st->nfalse_far += nfalse;
     if(nfalse > 0)
     {
         STARSH_int *false_far;
@@ -398,12 +400,11 @@ static size_t tile_bytes(int m, int n, int k, int maxrank)
             17*(size_t)k+(size_t)(m+n)*r+(size_t)m)+(size_t)n*k;
 }
 
-int hip_driver_run(const hip_problem *hp, STARSH_blrf *F, int maxrank,
-        double tol, STARSH_int ntiles, const STARSH_int *idx, const int *k,
-        int int8, double tau, hip_factors *hf, int *rank_out, int *flags_out,
-        int *accepted, hip_accept_fn accept, starsh_hip_stats *st)
+int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
+        const STARSH_int *idx, const int *k, int int8, double tau,
+        int *rank_out, int *flags_out, int *accepted, hip_accept_fn accept,
+        starsh_hip_stats *st)
 {
-    STARSH_cluster *RC = F->row_cluster, *CC = F->col_cluster;
     if(ntiles == 0)
         return STARSH_SUCCESS;
     int info = STARSH_SUCCESS;
@@ -413,10 +414,25 @@ int hip_driver_run(const hip_problem *hp, STARSH_blrf *F, int maxrank,
     int64_t *c0 = malloc(sizeof(int64_t)*ntiles);
     int64_t *uo = malloc(sizeof(int64_t)*ntiles);
     int64_t *vo = malloc(sizeof(int64_t)*ntiles);
-    if(!m || !n || !pp || !kc || !r0 || !c0 || !uo || !vo)
+    RsddProblem *probs = malloc(sizeof(RsddProblem)*B->nprob);
+    if(!m || !n || !pp || !kc || !r0 || !c0 || !uo || !vo || !probs)
         info = STARSH_MALLOC_ERROR;
+    for(int q = 0; info == STARSH_SUCCESS && q < B->nprob; q++)
+    {
+        const hip_problem *hp = B->hp+q;
+        probs[q].kd = &hp->kd;
+        probs[q].d_xr = hp->d_xr;
+        probs[q].d_xc = hp->d_xc;
+        probs[q].nr = hp->nr;
+        probs[q].nc = hp->nc;
+        probs[q].maxrank = B->maxrank[q];
+        probs[q].oversample = hp->oversample;
+        probs[q].tol = B->tol[q];
+    }
     for(STARSH_int t = 0; info == STARSH_SUCCESS && t < ntiles; t++)
     {
+        const STARSH_blrf *F = B->F[pidx[t]];
+        const STARSH_cluster *RC = F->row_cluster, *CC = F->col_cluster;
         STARSH_int bi = idx[t];
         STARSH_int i = F->block_far[2*bi], j = F->block_far[2*bi+1];
         m[t] = (int)RC->size[i];
@@ -434,14 +450,16 @@ int hip_driver_run(const hip_problem *hp, STARSH_blrf *F, int maxrank,
         STARSH_int b1 = b0;
         while(b1 < ntiles)
         {
+            const int q = pidx[b1];
+            const int mr = B->maxrank[q], os = B->hp[q].oversample;
             int mn = m[b1] < n[b1] ? m[b1] : n[b1];
             int kk = k[b1] < mn ? k[b1] : mn;
-            size_t tb = tile_bytes(m[b1], n[b1], kk, maxrank);
+            size_t tb = tile_bytes(m[b1], n[b1], kk, mr);
             if(b1 > b0 && bytes+tb > hip_driver_workspace_budget())
                 break;
             bytes += tb;
             st->flops_algorithmic += 4.0*m[b1]*(double)n[b1]*
-                (maxrank+hp->oversample < mn ? maxrank+hp->oversample : mn);
+                (mr+os < mn ? mr+os : mn);
             st->sketch_bytes_or_flops += 2.0*m[b1]*(double)n[b1]*kk;
             b1++;
         }
@@ -451,16 +469,11 @@ int hip_driver_run(const hip_problem *hp, STARSH_blrf *F, int maxrank,
         in.m = m+b0;
         in.n = n+b0;
         in.k = k+b0;
-        in.kd = &hp->kd;
-        in.d_xr = hp->d_xr;
-        in.d_xc = hp->d_xc;
-        in.nr = hp->nr;
-        in.nc = hp->nc;
+        in.nprob = B->nprob;
+        in.probs = probs;
+        in.prob = pidx+b0;
         in.r0 = r0+b0;
         in.c0 = c0+b0;
-        in.maxrank = maxrank;
-        in.oversample = hp->oversample;
-        in.tol = tol;
         in.sketch_int8 = int8;
         in.tau = tau;
         RsddOutput out;
@@ -480,15 +493,16 @@ int hip_driver_run(const hip_problem *hp, STARSH_blrf *F, int maxrank,
         double tc = hip_driver_now();
         for(STARSH_int t = b0; t < b1 && info == STARSH_SUCCESS; t++)
         {
-            int kk = k[t] < (m[t] < n[t] ? m[t] : n[t]) ? k[t] :
-                (m[t] < n[t] ? m[t] : n[t]);
-            int kfull = maxrank+hp->oversample;
-            int mn = m[t] < n[t] ? m[t] : n[t];
+            const int q = pidx[t];
+            const int mn = m[t] < n[t] ? m[t] : n[t];
+            const int kk = k[t] < mn ? k[t] : mn;
+            int kfull = B->maxrank[q]+B->hp[q].oversample;
             kfull = kfull < mn ? kfull : mn;
             accepted[t] = accept ? accept(rank_out[t], kk, kfull,
-                    flags_out[t], hp->oversample) : 1;
+                    flags_out[t], B->hp[q].oversample) : 1;
             if(!accepted[t])
                 continue;
+            hip_factors *hf = B->hf+q;
             STARSH_int bi = idx[t];
             hf->far_rank[bi] = rank_out[t];
             if(rank_out[t] <= 0)
@@ -505,20 +519,22 @@ int hip_driver_run(const hip_problem *hp, STARSH_blrf *F, int maxrank,
         b0 = b1;
     }
     free(m); free(n); free(pp); free(kc); free(r0); free(c0); free(uo);
-    free(vo);
+    free(vo); free(probs);
     return info;
 }
 
-void hip_driver_rank_stats(const hip_factors *hf, STARSH_int nfar,
-        starsh_hip_stats *st)
+void hip_driver_rank_stats(const hip_batch *B, starsh_hip_stats *st)
 {
-    long long cnt = 0, sum = 0;
+    long long cnt = 0, sum = 0, nfar = 0;
     st->rank_min = 1 << 30;
     st->rank_max = -1;
-    for(STARSH_int bi = 0; bi < nfar; bi++)
-        if(hf->far_rank[bi] >= 0)
+    for(int q = 0; q < B->nprob; q++)
+        for(STARSH_int bi = 0; bi < B->F[q]->nblocks_far; bi++)
         {
-            int r = hf->far_rank[bi];
+            nfar++;
+            const int r = B->hf[q].far_rank[bi];
+            if(r < 0)
+                continue;
             cnt++;
             sum += r;
             st->rank_min = r < st->rank_min ? r : st->rank_min;
@@ -530,48 +546,183 @@ void hip_driver_rank_stats(const hip_factors *hf, STARSH_int nfar,
     st->ntiles = nfar;
 }
 
-int starsh_blrm__drsdd_hip(STARSH_blrm **matrix, STARSH_blrf *format,
-        int maxrank, double tol, int onfly)
+/* Plain form over a batch: every far-field tile at the full sketch width
+ * k = min(maxrank+oversample, m, n). */
+int hip_plain_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
+        const STARSH_int *idx, starsh_hip_stats *st)
+{
+    int *k = malloc(sizeof(int)*(ntiles+1));
+    int *rk = malloc(sizeof(int)*(ntiles+1));
+    int *fl = malloc(sizeof(int)*(ntiles+1));
+    int *acc = malloc(sizeof(int)*(ntiles+1));
+    int info = STARSH_SUCCESS;
+    if(!k || !rk || !fl || !acc)
+        info = STARSH_MALLOC_ERROR;
+    for(STARSH_int t = 0; info == STARSH_SUCCESS && t < ntiles; t++)
+        k[t] = B->maxrank[pidx[t]]+B->hp[pidx[t]].oversample;
+    if(info == STARSH_SUCCESS)
+        info = hip_driver_run(B, ntiles, pidx, idx, k, 0, 0.0, rk, fl, acc,
+                NULL, st);
+    free(k); free(rk); free(fl); free(acc);
+    return info;
+}
+
+/* Kernel family and dimension of a prepared problem, as one sort key. */
+static int group_key(const hip_problem *hp)
+{
+    return hp->kd.family*16+hp->kd.ndim;
+}
+
+int hip_compress(int mixed, int nprob, STARSH_blrm **matrix,
+        STARSH_blrf **format, const int *maxrank, const double *tol,
+        int onfly)
 {
     double t_start = hip_driver_now();
-    if(matrix == NULL)
+    if(nprob <= 0 || matrix == NULL || format == NULL || maxrank == NULL ||
+            tol == NULL)
     {
-        STARSH_ERROR("Invalid value of `matrix`");
+        STARSH_ERROR("Invalid arguments");
         return STARSH_WRONG_PARAMETER;
     }
-    hip_problem hp;
-    int info = hip_driver_prepare(format, maxrank, tol, onfly, &hp);
-    if(info != STARSH_SUCCESS)
-        return info;
-    STARSH_blrf *F = format;
-    STARSH_int nfar = F->nblocks_far;
-    hip_factors hf;
-    info = hip_driver_alloc_factors(F, maxrank, &hf);
+    for(int q = 0; q < nprob; q++)
+        matrix[q] = NULL;
+    hip_batch B;
+    memset(&B, 0, sizeof(B));
+    B.nprob = nprob;
+    B.F = format;
+    B.maxrank = maxrank;
+    B.tol = tol;
+    B.hp = calloc(nprob, sizeof(hip_problem));
+    B.hf = calloc(nprob, sizeof(hip_factors));
     starsh_hip_stats st;
     memset(&st, 0, sizeof(st));
-    STARSH_int *idx = malloc(sizeof(STARSH_int)*(nfar+1));
-    int *k = malloc(sizeof(int)*(nfar+1)), *rk = malloc(sizeof(int)*(nfar+1));
-    int *fl = malloc(sizeof(int)*(nfar+1)), *acc = malloc(sizeof(int)*
-            (nfar+1));
-    if(info == STARSH_SUCCESS && (!idx || !k || !rk || !fl || !acc))
-        info = STARSH_MALLOC_ERROR;
-    for(STARSH_int bi = 0; info == STARSH_SUCCESS && bi < nfar; bi++)
+    int info = B.hp && B.hf ? STARSH_SUCCESS : STARSH_MALLOC_ERROR;
+    int nprep = 0;
+    for(int q = 0; info == STARSH_SUCCESS && q < nprob; q++)
     {
-        idx[bi] = bi;
-        k[bi] = maxrank+hp.oversample;
+        info = hip_driver_prepare(format[q], maxrank[q], tol[q], onfly,
+                B.hp+q);
+        if(info != STARSH_SUCCESS)
+            break;
+        nprep++;
+        info = hip_driver_alloc_factors(format[q], maxrank[q], B.hf+q);
     }
+    /* all far-field tiles, grouped by kernel family and dimension */
+    STARSH_int ntot = 0;
+    for(int q = 0; info == STARSH_SUCCESS && q < nprob; q++)
+        ntot += format[q]->nblocks_far;
+    int *pidx = malloc(sizeof(int)*(ntot+1));
+    STARSH_int *idx = malloc(sizeof(STARSH_int)*(ntot+1));
+    int *done = calloc(nprob, sizeof(int));
+    if(info == STARSH_SUCCESS && (!pidx || !idx || !done))
+        info = STARSH_MALLOC_ERROR;
+    for(int q0 = 0; info == STARSH_SUCCESS && q0 < nprob; q0++)
+    {
+        if(done[q0])
+            continue;
+        const int key = group_key(B.hp+q0);
+        STARSH_int nt = 0;
+        for(int q = q0; q < nprob; q++)
+            if(!done[q] && group_key(B.hp+q) == key)
+            {
+                done[q] = 1;
+                for(STARSH_int bi = 0; bi < format[q]->nblocks_far; bi++)
+                {
+                    pidx[nt] = q;
+                    idx[nt++] = bi;
+                }
+            }
+        info = mixed ? hip_mixed_tiles(&B, nt, pidx, idx, &st) :
+            hip_plain_tiles(&B, nt, pidx, idx, &st);
+    }
+    free(pidx); free(idx); free(done);
     if(info == STARSH_SUCCESS)
-        info = hip_driver_run(&hp, F, maxrank, tol, nfar, idx, k, 0, 0.0,
-                &hf, rk, fl, acc, NULL, &st);
-    if(info == STARSH_SUCCESS)
-        hip_driver_rank_stats(&hf, nfar, &st);
-    free(idx); free(k); free(rk); free(fl); free(acc);
-    if(info == STARSH_SUCCESS)
-        info = hip_driver_finish(matrix, F, &hp, &hf, onfly, &st);
+        hip_driver_rank_stats(&B, &st);
+    for(int q = 0; info == STARSH_SUCCESS && q < nprob; q++)
+        info = hip_driver_finish(matrix+q, format[q], B.hp+q, B.hf+q, onfly,
+                &st);
     if(info != STARSH_SUCCESS)
-        hip_driver_free_factors(F, &hf);
-    hip_driver_release(&hp);
+        for(int q = 0; q < nprob; q++)
+        {
+            if(matrix[q])
+            {
+                starsh_blrm_free(matrix[q]);
+                matrix[q] = NULL;
+            }
+            else if(B.hf && q < nprep)
+                hip_driver_free_factors(format[q], B.hf+q);
+        }
+    for(int q = 0; B.hp && q < nprep; q++)
+        hip_driver_release(B.hp+q);
+    free(B.hp);
+    free(B.hf);
     st.wall_seconds = hip_driver_now()-t_start;
     hip_driver_set_stats(&st);
     return info;
+}
+
+int starsh_blrm__drsdd_hip(STARSH_blrm **matrix, STARSH_blrf *format,
+        int maxrank, double tol, int onfly)
+{
+    if(matrix == NULL || format == NULL)
+    {
+        STARSH_ERROR("Invalid value of `matrix` or `format`");
+        return STARSH_WRONG_PARAMETER;
+    }
+    return hip_compress(0, 1, matrix, &format, &maxrank, &tol, onfly);
+}
+
+int starsh_blrm__drsdd_hip_mixed(STARSH_blrm **matrix, STARSH_blrf *format,
+        int maxrank, double tol, int onfly)
+{
+    if(matrix == NULL || format == NULL)
+    {
+        STARSH_ERROR("Invalid value of `matrix` or `format`");
+        return STARSH_WRONG_PARAMETER;
+    }
+    return hip_compress(1, 1, matrix, &format, &maxrank, &tol, onfly);
+}
+
+int starsh_blrm_approximate_batch(STARSH_blrm **matrix, STARSH_blrf **format,
+        int nprob, const int *maxrank, const double *tol, int onfly)
+{
+    if(nprob < 0 || (nprob > 0 && (matrix == NULL || format == NULL ||
+                    maxrank == NULL || tol == NULL)))
+    {
+        STARSH_ERROR("Invalid arguments");
+        return STARSH_WRONG_PARAMETER;
+    }
+    if(nprob == 0)
+        return STARSH_SUCCESS;
+    for(int q = 0; q < nprob; q++)
+        if(format[q] == NULL)
+        {
+            STARSH_ERROR("Invalid value of `format[%d]`", q);
+            return STARSH_WRONG_PARAMETER;
+        }
+    if(starsh_blrm_approximate == starsh_blrm__drsdd_hip)
+        return hip_compress(0, nprob, matrix, format, maxrank, tol, onfly);
+    if(starsh_blrm_approximate == starsh_blrm__drsdd_hip_mixed)
+        return hip_compress(1, nprob, matrix, format, maxrank, tol, onfly);
+    if(starsh_blrm_approximate == NULL)
+    {
+        STARSH_ERROR("starsh_init() was not called");
+        return STARSH_WRONG_PARAMETER;
+    }
+    /* any other engine: one problem after the other */
+    for(int q = 0; q < nprob; q++)
+    {
+        int info = starsh_blrm_approximate(matrix+q, format[q], maxrank[q],
+                tol[q], onfly);
+        if(info != STARSH_SUCCESS)
+        {
+            for(int r = 0; r < q; r++)
+            {
+                starsh_blrm_free(matrix[r]);
+                matrix[r] = NULL;
+            }
+            return info;
+        }
+    }
+    return STARSH_SUCCESS;
 }

@@ -22,14 +22,118 @@
 static const int bucket_widths[] = {32, 64, 128, 256, 512};
 #define NBUCKETS ((int)(sizeof(bucket_widths)/sizeof(bucket_widths[0])))
 
-static int first_bucket = 0;
+/* First bucket forced by starsh_hip_set_option("mixed_first_bucket"), or -1
+ * for the schedule table (then 0 when the problem has no entry). */
+static int forced_bucket = -1;
+/* Tiles still pending after the first bucket of the last call. */
+static long long last_pending_after_first = 0;
 
 int hip_mixed_set_first_bucket(int b)
 {
-    if(b < 0 || b >= NBUCKETS)
+    if(b < -1 || b >= NBUCKETS)
         return STARSH_WRONG_PARAMETER;
-    first_bucket = b;
+    forced_bucket = b;
     return STARSH_SUCCESS;
+}
+
+/* Schedule table of the autotuner: the first bucket that gave the shortest
+ * compression time for a problem signature (kernel, its parameters, point
+ * count, largest cluster, maxrank, oversample, tolerance).  Entries come
+ * from starsh_hip_autotune() and from the text file named by the
+ * environment variable STARSH_HIP_SCHEDULE, one entry per line:
+ *   family ndim beta nu noise sigma wave_k diag n nb maxrank os tol first */
+typedef struct sched_entry
+{
+    starsh_hip_kernel_desc kd;
+    long long n;
+    int nb, maxrank, os;
+    double tol;
+    int first;
+} sched_entry;
+
+#define SCHED_MAX 256
+static sched_entry sched[SCHED_MAX];
+static int nsched = 0, sched_loaded = 0;
+
+static void sched_key(const hip_problem *hp, const STARSH_blrf *F,
+        int maxrank, double tol, sched_entry *e)
+{
+    memset(e, 0, sizeof(*e));
+    e->kd = hp->kd;
+    e->n = hp->nr;
+    int nb = 0;
+    for(STARSH_int i = 0; i < F->row_cluster->nblocks; i++)
+        if(F->row_cluster->size[i] > nb)
+            nb = (int)F->row_cluster->size[i];
+    e->nb = nb;
+    e->maxrank = maxrank;
+    e->os = hp->oversample;
+    e->tol = tol;
+    e->first = -1;
+}
+
+static int sched_same(const sched_entry *a, const sched_entry *b)
+{
+    const starsh_hip_kernel_desc *x = &a->kd, *y = &b->kd;
+    return x->family == y->family && x->ndim == y->ndim &&
+        x->beta == y->beta && x->nu == y->nu && x->noise == y->noise &&
+        x->sigma == y->sigma && x->wave_k == y->wave_k &&
+        x->diag == y->diag && a->n == b->n && a->nb == b->nb &&
+        a->maxrank == b->maxrank && a->os == b->os && a->tol == b->tol;
+}
+
+static void sched_put(const sched_entry *e)
+{
+    for(int i = 0; i < nsched; i++)
+        if(sched_same(&sched[i], e))
+        {
+            sched[i].first = e->first;
+            return;
+        }
+    if(nsched < SCHED_MAX)
+        sched[nsched++] = *e;
+}
+
+static void sched_load(void)
+{
+    if(sched_loaded)
+        return;
+    sched_loaded = 1;
+    const char *path = getenv("STARSH_HIP_SCHEDULE");
+    FILE *f = path ? fopen(path, "r") : NULL;
+    if(f == NULL)
+        return;
+    sched_entry e;
+    memset(&e, 0, sizeof(e));
+    while(fscanf(f, "%d %d %lg %lg %lg %lg %lg %lg %lld %d %d %d %lg %d",
+                &e.kd.family, &e.kd.ndim, &e.kd.beta, &e.kd.nu, &e.kd.noise,
+                &e.kd.sigma, &e.kd.wave_k, &e.kd.diag, &e.n, &e.nb,
+                &e.maxrank, &e.os, &e.tol, &e.first) == 14)
+        if(e.first >= 0 && e.first < NBUCKETS)
+            sched_put(&e);
+    fclose(f);
+}
+
+static void sched_save(const sched_entry *e)
+{
+    const char *path = getenv("STARSH_HIP_SCHEDULE");
+    FILE *f = path ? fopen(path, "a") : NULL;
+    if(f == NULL)
+        return;
+    fprintf(f, "%d %d %.17g %.17g %.17g %.17g %.17g %.17g %lld %d %d %d "
+            "%.17g %d\n", e->kd.family, e->kd.ndim, e->kd.beta, e->kd.nu,
+            e->kd.noise, e->kd.sigma, e->kd.wave_k, e->kd.diag, e->n, e->nb,
+            e->maxrank, e->os, e->tol, e->first);
+    fclose(f);
+}
+
+static int sched_first(const sched_entry *key)
+{
+    sched_load();
+    for(int i = 0; i < nsched; i++)
+        if(sched_same(&sched[i], key))
+            return sched[i].first;
+    return -1;
 }
 
 /* A randomized range finder with k = r + p samples leaves an expected
@@ -48,87 +152,154 @@ static int accept_mixed(int rank, int k, int kfull, int flags, int os)
     return (flags & 1) != 0;
 }
 
-int starsh_blrm__drsdd_hip_mixed(STARSH_blrm **matrix, STARSH_blrf *format,
-        int maxrank, double tol, int onfly)
+int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
+        const STARSH_int *idx, starsh_hip_stats *st)
 {
-    double t_start = hip_driver_now();
-    if(matrix == NULL)
-    {
-        STARSH_ERROR("Invalid value of `matrix`");
-        return STARSH_WRONG_PARAMETER;
-    }
-    hip_problem hp;
-    int info = hip_driver_prepare(format, maxrank, tol, onfly, &hp);
-    if(info != STARSH_SUCCESS)
-        return info;
-    STARSH_blrf *F = format;
-    STARSH_cluster *RC = F->row_cluster, *CC = F->col_cluster;
-    STARSH_int nfar = F->nblocks_far;
-    hip_factors hf;
-    info = hip_driver_alloc_factors(F, maxrank, &hf);
-    starsh_hip_stats st;
-    memset(&st, 0, sizeof(st));
-    STARSH_int *pending = malloc(sizeof(STARSH_int)*(nfar+1));
-    STARSH_int *next = malloc(sizeof(STARSH_int)*(nfar+1));
-    int *k = malloc(sizeof(int)*(nfar+1)), *rk = malloc(sizeof(int)*(nfar+1));
-    int *fl = malloc(sizeof(int)*(nfar+1));
-    int *acc = malloc(sizeof(int)*(nfar+1));
-    if(info == STARSH_SUCCESS && (!pending || !next || !k || !rk || !fl ||
-                !acc))
+    if(ntiles == 0)
+        return STARSH_SUCCESS;
+    int info = STARSH_SUCCESS;
+    /* pending tiles (positions into pidx/idx) and their current bucket */
+    STARSH_int *pend = malloc(sizeof(STARSH_int)*ntiles);
+    STARSH_int *next = malloc(sizeof(STARSH_int)*ntiles);
+    int *bucket = malloc(sizeof(int)*ntiles);
+    int *pq = malloc(sizeof(int)*ntiles);
+    STARSH_int *pi = malloc(sizeof(STARSH_int)*ntiles);
+    int *k = malloc(sizeof(int)*ntiles), *rk = malloc(sizeof(int)*ntiles);
+    int *fl = malloc(sizeof(int)*ntiles), *acc = malloc(sizeof(int)*ntiles);
+    int *first = malloc(sizeof(int)*B->nprob);
+    if(!pend || !next || !bucket || !pq || !pi || !k || !rk || !fl || !acc ||
+            !first)
         info = STARSH_MALLOC_ERROR;
-    STARSH_int npending = nfar;
-    for(STARSH_int bi = 0; bi < nfar && info == STARSH_SUCCESS; bi++)
-        pending[bi] = bi;
     /* Deflation threshold of the sketch columns: above the int8 sketch's
-     * representation noise (2^-28 of a row's largest entry) and well below
-     * the requested accuracy. */
-    double tau = 0.1*tol;
+     * representation noise (2^-28 of a chunk's largest entry) and well
+     * below the requested accuracy (the tightest one of the batch). */
+    double tmin = 1e300;
+    for(int q = 0; info == STARSH_SUCCESS && q < B->nprob; q++)
+        tmin = B->tol[q] < tmin ? B->tol[q] : tmin;
+    double tau = 0.1*tmin;
     tau = tau > 1e-8 ? 1e-8 : tau;
     tau = tau < 1e-13 ? 1e-13 : tau;
-    /* Four 7-bit digits represent A to 2^-28 of each row's largest entry;
+    /* Four 7-bit digits represent A to 2^-28 of a chunk's largest entry;
      * tighter tolerances keep the FP64 sketch (still rank-bucketed). */
-    int int8 = tol >= 3e-8;
+    const int int8 = tmin >= 3e-8;
     if(!int8)
         tau = 1e-13;
-    for(int b = first_bucket; b < NBUCKETS && npending > 0 &&
-            info == STARSH_SUCCESS; b++)
+    for(int q = 0; info == STARSH_SUCCESS && q < B->nprob; q++)
     {
-        int last = 1;
-        for(STARSH_int t = 0; t < npending; t++)
+        first[q] = forced_bucket;
+        if(first[q] < 0)
         {
-            STARSH_int bi = pending[t];
-            int m = (int)RC->size[F->block_far[2*bi]];
-            int n = (int)CC->size[F->block_far[2*bi+1]];
-            int mn = m < n ? m : n;
-            int kfull = maxrank+hp.oversample < mn ? maxrank+hp.oversample :
-                mn;
-            k[t] = bucket_widths[b] < kfull ? bucket_widths[b] : kfull;
-            if(b == NBUCKETS-1)
-                k[t] = kfull;
-            if(k[t] < kfull)
-                last = 0;
+            sched_entry key;
+            sched_key(B->hp+q, B->F[q], B->maxrank[q], B->tol[q], &key);
+            first[q] = sched_first(&key);
+            if(first[q] < 0)
+                first[q] = 0;
         }
-        (void)last;
-        info = hip_driver_run(&hp, F, maxrank, tol, npending, pending, k,
-                int8, tau, &hf, rk, fl, acc, accept_mixed, &st);
-        STARSH_int nn = 0;
-        for(STARSH_int t = 0; t < npending && info == STARSH_SUCCESS; t++)
-            if(!acc[t])
-                next[nn++] = pending[t];
-        STARSH_int *tmp = pending;
-        pending = next;
-        next = tmp;
-        npending = nn;
     }
-    if(info == STARSH_SUCCESS)
-        hip_driver_rank_stats(&hf, nfar, &st);
-    free(pending); free(next); free(k); free(rk); free(fl); free(acc);
-    if(info == STARSH_SUCCESS)
-        info = hip_driver_finish(matrix, F, &hp, &hf, onfly, &st);
-    if(info != STARSH_SUCCESS)
-        hip_driver_free_factors(F, &hf);
-    hip_driver_release(&hp);
-    st.wall_seconds = hip_driver_now()-t_start;
-    hip_driver_set_stats(&st);
+    STARSH_int npend = info == STARSH_SUCCESS ? ntiles : 0;
+    for(STARSH_int t = 0; t < npend; t++)
+    {
+        pend[t] = t;
+        bucket[t] = first[pidx[t]];
+    }
+    last_pending_after_first = npend;
+    for(int round = 0; npend > 0 && info == STARSH_SUCCESS; round++)
+    {
+        for(STARSH_int u = 0; u < npend; u++)
+        {
+            const STARSH_int t = pend[u];
+            const int q = pidx[t];
+            const STARSH_blrf *F = B->F[q];
+            const STARSH_int bi = idx[t];
+            const int m = (int)F->row_cluster->size[F->block_far[2*bi]];
+            const int n = (int)F->col_cluster->size[F->block_far[2*bi+1]];
+            const int mn = m < n ? m : n;
+            const int os = B->hp[q].oversample;
+            const int kfull = B->maxrank[q]+os < mn ? B->maxrank[q]+os : mn;
+            const int b = bucket[t];
+            k[u] = b >= NBUCKETS-1 || bucket_widths[b] >= kfull ? kfull :
+                bucket_widths[b];
+            pq[u] = q;
+            pi[u] = bi;
+        }
+        info = hip_driver_run(B, npend, pq, pi, k, int8, tau, rk, fl, acc,
+                accept_mixed, st);
+        STARSH_int nn = 0;
+        for(STARSH_int u = 0; u < npend && info == STARSH_SUCCESS; u++)
+            if(!acc[u])
+            {
+                const STARSH_int t = pend[u];
+                bucket[t]++;
+                next[nn++] = t;
+            }
+        STARSH_int *tmp = pend;
+        pend = next;
+        next = tmp;
+        npend = nn;
+        if(round == 0)
+            last_pending_after_first = nn;
+    }
+    free(pend); free(next); free(bucket); free(pq); free(pi); free(k);
+    free(rk); free(fl); free(acc); free(first);
     return info;
+}
+
+/* Runs the mixed form once per candidate first bucket (after one untimed
+ * run), keeps the fastest by device time, records it in the schedule table
+ * (and in the STARSH_HIP_SCHEDULE file) and returns it in *first_out.  The
+ * candidates stop at the first bucket that finishes every tile. */
+int starsh_hip_autotune(STARSH_blrf *format, int maxrank, double tol,
+        int onfly, int *first_out, double *seconds_out)
+{
+    if(format == NULL)
+        return STARSH_WRONG_PARAMETER;
+    const int saved = forced_bucket;
+    STARSH_blrm *M = NULL;
+    forced_bucket = 0;
+    int info = starsh_blrm__drsdd_hip_mixed(&M, format, maxrank, tol, onfly);
+    if(info != STARSH_SUCCESS)
+    {
+        forced_bucket = saved;
+        return info;
+    }
+    starsh_blrm_free(M);
+    int best = 0;
+    double best_t = 1e300;
+    for(int b = 0; b < NBUCKETS; b++)
+    {
+        forced_bucket = b;
+        M = NULL;
+        info = starsh_blrm__drsdd_hip_mixed(&M, format, maxrank, tol, onfly);
+        if(info != STARSH_SUCCESS)
+            break;
+        starsh_hip_stats st;
+        starsh_hip_get_stats(&st);
+        starsh_blrm_free(M);
+        if(seconds_out)
+            seconds_out[b] = st.device_seconds;
+        if(st.device_seconds < best_t)
+        {
+            best_t = st.device_seconds;
+            best = b;
+        }
+        if(last_pending_after_first == 0)
+            break;
+    }
+    forced_bucket = saved;
+    if(info != STARSH_SUCCESS)
+        return info;
+    hip_problem hp;
+    info = hip_driver_prepare(format, maxrank, tol, onfly, &hp);
+    if(info != STARSH_SUCCESS)
+        return info;
+    sched_entry e;
+    sched_key(&hp, format, maxrank, tol, &e);
+    hip_driver_release(&hp);
+    e.first = best;
+    sched_load();
+    sched_put(&e);
+    sched_save(&e);
+    if(first_out)
+        *first_out = best;
+    return STARSH_SUCCESS;
 }

@@ -16,7 +16,7 @@
 namespace {
 
 constexpr int CORE_THREADS = 256;
-constexpr int LDS_PMAX = 120;
+constexpr int LDS_PMAX = 128;
 
 __device__ inline double group_sum(double v, int g)
 {
@@ -100,15 +100,20 @@ __device__ void jacobi(double *W, int ldw, int p, int *sweeps_out)
     *sweeps_out = sweep;
 }
 
+// Dynamic LDS: sig[pmax] doubles, then (when lds_p > 0) W[lds_p^2]
+// doubles, then perm[pmax] ints.  Tiles with p <= lds_p work in LDS, larger
+// ones in their workspace.
 __global__ __launch_bounds__(CORE_THREADS) void core_kernel(TileDesc *tiles,
-        double *ws, int maxrank, double tol)
+        double *ws, int pmax, int lds_p)
 {
-    __shared__ double lds[LDS_PMAX*LDS_PMAX];
-    __shared__ double sig[1024];
-    __shared__ int perm[1024];
+    extern __shared__ double dyn[];
+    double *sig = dyn;
+    double *lds = dyn+pmax;
+    int *perm = (int *)(lds+(int64_t)lds_p*lds_p);
     __shared__ int s_rank;
     TileDesc &td = tiles[blockIdx.x];
-    const int p = td.p, k = td.k;
+    const int p = td.p, k = td.k, maxrank = td.maxrank;
+    const double tol = td.tol;
     if(p <= 0)
     {
         // zero sketch: all singular values vanish and the rank is 0
@@ -121,7 +126,7 @@ __global__ __launch_bounds__(CORE_THREADS) void core_kernel(TileDesc *tiles,
     const double *R2 = ws+td.r_off;
     double *W;
     int ldw;
-    if(p <= LDS_PMAX)
+    if(p <= lds_p)
     {
         W = lds;
         ldw = p;
@@ -227,15 +232,28 @@ __global__ __launch_bounds__(CORE_THREADS) void core_kernel(TileDesc *tiles,
 
 } // namespace
 
-int pipeline_core(TileDesc *d_tiles, int ntiles, double *ws, int maxrank,
-        double tol, int pmax, hipStream_t s)
+int pipeline_core(TileDesc *d_tiles, int ntiles, double *ws, int pmax,
+        hipStream_t s)
 {
     if(ntiles <= 0)
         return STARSH_SUCCESS;
     if(pmax > 1024)
         return STARSH_WRONG_PARAMETER;
-    hipLaunchKernelGGL(core_kernel, dim3(ntiles), dim3(CORE_THREADS), 0, s,
-            d_tiles, ws, maxrank, tol);
+    const int pm = pmax > 0 ? pmax : 1;
+    const int lds_p = pm <= LDS_PMAX ? pm : LDS_PMAX;
+    const size_t bytes = sizeof(double)*((size_t)pm+(size_t)lds_p*lds_p)+
+        sizeof(int)*(size_t)pm;
+    static bool attr = false;
+    if(!attr)
+    {
+        HIP_TRY(hipFuncSetAttribute((const void *)core_kernel,
+                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                    (int)(sizeof(double)*(1024+LDS_PMAX*LDS_PMAX)+
+                        sizeof(int)*1024)));
+        attr = true;
+    }
+    hipLaunchKernelGGL(core_kernel, dim3(ntiles), dim3(CORE_THREADS), bytes,
+            s, d_tiles, ws, pm, lds_p);
     HIP_TRY(hipGetLastError());
     return STARSH_SUCCESS;
 }

@@ -4,10 +4,9 @@
 // blocks outside a tile's own output return at once.
 //
 // Modes (C = alpha*op(A)*op(B) + beta*C):
-//   SKETCH_GEN  Y[m x k]  = A[m x n] Omega[n x k], A generated from points
-//   SKETCH_MEM  same with A explicit in memory
-//   PROJ_GEN    Bt[n x kc] = A^T[n x m] Q[m x kc], A generated
-//   PROJ_MEM    same with A explicit
+//   SKETCH_MEM  Y[m x k]  = A[m x n] Omega[n x k], A explicit in memory
+//   PROJ_MEM    Bt[n x kc] = A^T[n x m] Q[m x kc], A explicit
+// (tiles generated from points use gen_gemm.hip instead)
 //   COEF        C[j0 x jb] = Q[:, :j0]^T W, W = Q[:, j0:j0+jb]
 //   UPDATE      W -= Q[:, :j0] C
 //   R2          R2[p x p] = Q2^T Bt[:, :p]
@@ -24,13 +23,12 @@ constexpr int BM = 64, BN = 64, BK = 16, LDA_S = BM+2, LDB_S = BN+2;
 
 struct Op
 {
-    const double *A;   // memory operand A (unused when generated)
+    const double *A;
     const double *B;
     double *C;
     int M, N, K;
     int lda, ldb, ldc;
     double alpha, beta;
-    int64_t gr0, gc0;  // generated operand: row / column point offsets
     bool valid;
 };
 
@@ -41,8 +39,6 @@ __device__ inline Op setup(const TileDesc &t, const GemmArgs &g)
     o.valid = true;
     o.alpha = 1.0;
     o.beta = 0.0;
-    o.gr0 = t.r0;
-    o.gc0 = t.c0;
     double *ws = g.ws;
     const bool q2 = g.which != 0;
     const int rows = q2 ? t.n : t.m;
@@ -50,14 +46,12 @@ __device__ inline Op setup(const TileDesc &t, const GemmArgs &g)
     const int width = q2 ? t.p : t.k;
     switch(MODE)
     {
-        case GM_SKETCH_GEN:
         case GM_SKETCH_MEM:
             o.M = t.m; o.N = t.k; o.K = t.n;
             o.A = g.abase+t.a_off; o.lda = t.m;
             o.B = ws+t.om_off; o.ldb = t.n;
             o.C = ws+t.q_off; o.ldc = t.m;
             break;
-        case GM_PROJ_GEN:
         case GM_PROJ_MEM:
             o.M = t.n; o.N = t.kc; o.K = t.m;
             o.A = g.abase+t.a_off; o.lda = t.m;
@@ -112,12 +106,9 @@ __device__ inline Op setup(const TileDesc &t, const GemmArgs &g)
 }
 
 // A operand layouts: 0 = column-major (element (i,k) at A[k*lda+i]),
-// 1 = transposed storage (element (i,k) at A[i*lda+k]), 2 = generated A,
-// 3 = generated A^T.
+// 1 = transposed storage (element (i,k) at A[i*lda+k]).
 template <int MODE> struct ModeTraits;
-template <> struct ModeTraits<GM_SKETCH_GEN> { static constexpr int a = 2; };
 template <> struct ModeTraits<GM_SKETCH_MEM> { static constexpr int a = 0; };
-template <> struct ModeTraits<GM_PROJ_GEN> { static constexpr int a = 3; };
 template <> struct ModeTraits<GM_PROJ_MEM> { static constexpr int a = 1; };
 template <> struct ModeTraits<GM_COEF> { static constexpr int a = 1; };
 template <> struct ModeTraits<GM_UPDATE> { static constexpr int a = 0; };
@@ -125,7 +116,7 @@ template <> struct ModeTraits<GM_R2> { static constexpr int a = 1; };
 template <> struct ModeTraits<GM_FORM_U> { static constexpr int a = 0; };
 template <> struct ModeTraits<GM_FORM_V> { static constexpr int a = 0; };
 
-template <int MODE, int NDIM>
+template <int MODE>
 __global__ __launch_bounds__(256) void gemm_kernel(const TileDesc *tiles,
         GemmArgs g)
 {
@@ -155,7 +146,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(const TileDesc *tiles,
     for(int k0 = 0; k0 < o.K; k0 += BK)
     {
         // ---- stage A (BM x BK) as As[k][i]
-        if(AL == 0 || AL == 2 || AL == 3)
+        if(AL == 0)
         {
             const int i = tid & 63, gi = row0+i;
 #pragma unroll
@@ -164,16 +155,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(const TileDesc *tiles,
                 const int kk = (tid >> 6)+4*s, gk = k0+kk;
                 double v = 0.0;
                 if(gi < o.M && gk < o.K)
-                {
-                    if(AL == 0)
-                        v = o.A[(int64_t)gk*o.lda+gi];
-                    else if(AL == 2)
-                        v = kernel_value<NDIM>(g.kp, g.ps.xr, g.ps.nr,
-                                o.gr0+gi, g.ps.xc, g.ps.nc, o.gc0+gk);
-                    else
-                        v = kernel_value<NDIM>(g.kp, g.ps.xr, g.ps.nr,
-                                o.gr0+gk, g.ps.xc, g.ps.nc, o.gc0+gi);
-                }
+                    v = o.A[(int64_t)gk*o.lda+gi];
                 As[kk][i] = v;
             }
         }
@@ -246,31 +228,17 @@ __global__ __launch_bounds__(256) void gemm_kernel(const TileDesc *tiles,
         }
 }
 
-template <int MODE, int NDIM>
-int launch_ndim(const TileDesc *d_tiles, int ntiles, int mmax, int nmax,
+template <int MODE>
+int launch(const TileDesc *d_tiles, int ntiles, int mmax, int nmax,
         const GemmArgs &g, hipStream_t s)
 {
     int nbm = (mmax+BM-1)/BM, nbn = (nmax+BN-1)/BN;
     if(nbm <= 0 || nbn <= 0 || ntiles <= 0)
         return STARSH_SUCCESS;
-    hipLaunchKernelGGL((gemm_kernel<MODE, NDIM>), dim3(nbm*nbn, ntiles),
+    hipLaunchKernelGGL((gemm_kernel<MODE>), dim3(nbm*nbn, ntiles),
             dim3(256), 0, s, d_tiles, g);
     HIP_TRY(hipGetLastError());
     return STARSH_SUCCESS;
-}
-
-template <int MODE>
-int launch(const TileDesc *d_tiles, int ntiles, int mmax, int nmax,
-        const GemmArgs &g, hipStream_t s)
-{
-    switch(g.kp.ndim)
-    {
-        case 1: return launch_ndim<MODE, 1>(d_tiles, ntiles, mmax, nmax, g, s);
-        case 2: return launch_ndim<MODE, 2>(d_tiles, ntiles, mmax, nmax, g, s);
-        case 3: return launch_ndim<MODE, 3>(d_tiles, ntiles, mmax, nmax, g, s);
-        default: return launch_ndim<MODE, 4>(d_tiles, ntiles, mmax, nmax, g,
-                         s);
-    }
 }
 
 } // namespace
@@ -280,24 +248,19 @@ int pipeline_gemm(int mode, const TileDesc *d_tiles, int ntiles, int mmax,
 {
     switch(mode)
     {
-        case GM_SKETCH_GEN: return launch<GM_SKETCH_GEN>(d_tiles, ntiles, mmax,
-                                    nmax, g, s);
-        case GM_SKETCH_MEM: return launch_ndim<GM_SKETCH_MEM, 1>(d_tiles,
-                                    ntiles, mmax, nmax, g, s);
-        case GM_PROJ_GEN: return launch<GM_PROJ_GEN>(d_tiles, ntiles, mmax,
+        case GM_SKETCH_MEM: return launch<GM_SKETCH_MEM>(d_tiles, ntiles,
+                                    mmax, nmax, g, s);
+        case GM_PROJ_MEM: return launch<GM_PROJ_MEM>(d_tiles, ntiles, mmax,
                                   nmax, g, s);
-        case GM_PROJ_MEM: return launch_ndim<GM_PROJ_MEM, 1>(d_tiles, ntiles,
-                                  mmax, nmax, g, s);
-        case GM_COEF: return launch_ndim<GM_COEF, 1>(d_tiles, ntiles, mmax,
-                              nmax, g, s);
-        case GM_UPDATE: return launch_ndim<GM_UPDATE, 1>(d_tiles, ntiles, mmax,
-                                nmax, g, s);
-        case GM_R2: return launch_ndim<GM_R2, 1>(d_tiles, ntiles, mmax, nmax,
-                            g, s);
-        case GM_FORM_U: return launch_ndim<GM_FORM_U, 1>(d_tiles, ntiles, mmax,
-                                nmax, g, s);
-        case GM_FORM_V: return launch_ndim<GM_FORM_V, 1>(d_tiles, ntiles, mmax,
-                                nmax, g, s);
+        case GM_COEF: return launch<GM_COEF>(d_tiles, ntiles, mmax, nmax, g,
+                              s);
+        case GM_UPDATE: return launch<GM_UPDATE>(d_tiles, ntiles, mmax, nmax,
+                                g, s);
+        case GM_R2: return launch<GM_R2>(d_tiles, ntiles, mmax, nmax, g, s);
+        case GM_FORM_U: return launch<GM_FORM_U>(d_tiles, ntiles, mmax, nmax,
+                                g, s);
+        case GM_FORM_V: return launch<GM_FORM_V>(d_tiles, ntiles, mmax, nmax,
+                                g, s);
         default: return STARSH_WRONG_PARAMETER;
     }
 }

@@ -1,0 +1,218 @@
+// FP64 MFMA GEMMs whose left operand is a kernel tile generated on the fly:
+//   SKETCH  Y[m x k]  = A[m x n] Omega[n x k]
+//   PROJ    Bt[n x kc] = A^T[n x m] Q[m x kc]
+// A never exists in memory.  Each wave owns 16 rows of the block and all BN
+// columns, so every generated entry is produced exactly once per block and
+// feeds BN/16 MFMAs; lane l generates A[row = l%16][k = l/16] directly in
+// the A-operand layout of v_mfma_f64_16x16x4f64.  The right operand is
+// staged through LDS, double buffered, one barrier per K step of 16.
+#include "pipeline.h"
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int BM = 64;
+constexpr int BK = 16;
+
+// Column block width (0: by problem size); set by starsh_hip_set_option.
+int gen_gemm_bn = 0;
+
+// PROJ=false: rows are row points, K runs over column points.
+// PROJ=true : rows are column points, K runs over row points (A^T).
+template <int NDIM, int FAM, int BN, bool PROJ>
+__global__ __launch_bounds__(256) void gen_gemm_kernel(const TileDesc *tiles,
+        GemmArgs g)
+{
+    constexpr int NF = BN/16;
+    const TileDesc t = tiles[blockIdx.y];
+    int M, N, K, ldb, ldc;
+    const double *B;
+    double *C;
+    int64_t rbase, kbase;
+    const double *xrow, *xk;
+    int64_t nrow, nk;
+    if(!PROJ)
+    {
+        M = t.m; N = t.k; K = t.n;
+        B = g.ws+t.om_off; ldb = t.n;
+        C = g.ws+t.q_off; ldc = t.m;
+        rbase = t.r0; kbase = t.c0;
+        xrow = t.xr; nrow = t.nr;
+        xk = t.xc; nk = t.nc;
+    }
+    else
+    {
+        M = t.n; N = t.kc; K = t.m;
+        B = g.ws+t.q_off; ldb = t.m;
+        C = g.ws+t.bt_off; ldc = t.n;
+        rbase = t.c0; kbase = t.r0;
+        xrow = t.xc; nrow = t.nc;
+        xk = t.xr; nk = t.nr;
+    }
+    if(M <= 0 || N <= 0)
+        return;
+    const int nbm = (M+BM-1)/BM;
+    const int bm = blockIdx.x % nbm, bn = blockIdx.x / nbm;
+    if(bn*BN >= N)
+        return;
+    const int row0 = bm*BM, col0 = bn*BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+    __shared__ double Bs[2][BK][BN+2];
+
+    // this lane's generated row (fixed) and its K offset inside a 4-step
+    const int grow = row0+wave*16+(lane & 15);
+    const bool row_ok = grow < M;
+    double xi[NDIM];
+#pragma unroll
+    for(int d = 0; d < NDIM; d++)
+        xi[d] = row_ok ? xrow[d*nrow+rbase+grow] : 0.0;
+    const int kl = lane >> 4;
+
+    double4_t acc[NF];
+#pragma unroll
+    for(int f = 0; f < NF; f++)
+        acc[f] = double4_t{0.0, 0.0, 0.0, 0.0};
+
+    // B staging: thread loads (kk = tid%16, j = tid/16 + 16 s)
+    const int bk = tid & 15, bj = tid >> 4;
+    double breg[NF];
+    double areg[4];
+
+    auto load_b = [&](int k0) {
+#pragma unroll
+        for(int s = 0; s < NF; s++)
+        {
+            const int j = col0+bj+16*s, k = k0+bk;
+            breg[s] = (j < N && k < K) ? B[(int64_t)j*ldb+k] : 0.0;
+        }
+    };
+    auto store_b = [&](int buf) {
+#pragma unroll
+        for(int s = 0; s < NF; s++)
+            Bs[buf][bk][bj+16*s] = breg[s];
+    };
+    auto gen_a = [&](int k0) {
+#pragma unroll
+        for(int q = 0; q < 4; q++)
+        {
+            const int k = k0+4*q+kl;
+            areg[q] = (row_ok && k < K) ? kval<NDIM, FAM>(t.kp, xi, xk, nk,
+                    kbase+k) : 0.0;
+        }
+    };
+
+    load_b(0);
+    store_b(0);
+    gen_a(0);
+    __syncthreads();
+    int buf = 0;
+    for(int k0 = 0; k0 < K; k0 += BK)
+    {
+        double acur[4];
+#pragma unroll
+        for(int q = 0; q < 4; q++)
+            acur[q] = areg[q];
+        const bool more = k0+BK < K;
+        if(more)
+        {
+            load_b(k0+BK);
+            gen_a(k0+BK);
+        }
+#pragma unroll
+        for(int q = 0; q < 4; q++)
+        {
+#pragma unroll
+            for(int f = 0; f < NF; f++)
+            {
+                const double b = Bs[buf][4*q+kl][f*16+(lane & 15)];
+                acc[f] = __builtin_amdgcn_mfma_f64_16x16x4f64(acur[q], b,
+                        acc[f], 0, 0, 0);
+            }
+        }
+        if(more)
+            store_b(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    // epilogue: lane holds rows lane/16 + 4 v, column lane%16 of each frag
+#pragma unroll
+    for(int f = 0; f < NF; f++)
+    {
+        const int gj = col0+f*16+(lane & 15);
+        if(gj >= N)
+            continue;
+#pragma unroll
+        for(int v = 0; v < 4; v++)
+        {
+            const int gi = row0+wave*16+(lane >> 4)+4*v;
+            if(gi < M)
+                C[(int64_t)gj*ldc+gi] = acc[f][v];
+        }
+    }
+}
+
+template <int NDIM, int FAM, bool PROJ>
+int launch_nd(const TileDesc *d_tiles, int ntiles, int mmax, int nmax,
+        const GemmArgs &g, hipStream_t s)
+{
+    const int nbm = (mmax+BM-1)/BM;
+    int bn = gen_gemm_bn > 0 ? gen_gemm_bn : (nmax > 128 ? 256 :
+            (nmax > 64 ? 128 : 64));
+    if(bn == 256)
+    {
+        dim3 grid(nbm*((nmax+255)/256), ntiles);
+        hipLaunchKernelGGL((gen_gemm_kernel<NDIM, FAM, 256, PROJ>), grid,
+                dim3(256), 0, s, d_tiles, g);
+    }
+    else if(bn == 128)
+    {
+        dim3 grid(nbm*((nmax+127)/128), ntiles);
+        hipLaunchKernelGGL((gen_gemm_kernel<NDIM, FAM, 128, PROJ>), grid,
+                dim3(256), 0, s, d_tiles, g);
+    }
+    else
+    {
+        dim3 grid(nbm*((nmax+63)/64), ntiles);
+        hipLaunchKernelGGL((gen_gemm_kernel<NDIM, FAM, 64, PROJ>), grid,
+                dim3(256), 0, s, d_tiles, g);
+    }
+    HIP_TRY(hipGetLastError());
+    return STARSH_SUCCESS;
+}
+
+template <bool PROJ>
+struct GenLaunch
+{
+    const TileDesc *tiles;
+    int ntiles, mmax, nmax;
+    const GemmArgs &g;
+    hipStream_t s;
+    template <int NDIM, int FAM> int operator()() const
+    {
+        return launch_nd<NDIM, FAM, PROJ>(tiles, ntiles, mmax, nmax, g, s);
+    }
+};
+
+} // namespace
+
+extern "C" int starsh_hip_gen_gemm_bn(int bn)
+{
+    if(bn != 0 && bn != 64 && bn != 128 && bn != 256)
+        return STARSH_WRONG_PARAMETER;
+    gen_gemm_bn = bn;
+    return STARSH_SUCCESS;
+}
+
+int pipeline_gen_gemm(int proj, const TileDesc *d_tiles, int ntiles,
+        int mmax, int nmax, const GemmArgs &g, hipStream_t s)
+{
+    if(ntiles <= 0 || mmax <= 0 || nmax <= 0)
+        return STARSH_SUCCESS;
+    if(proj)
+        return dispatch_kernel(g.ndim, g.family, GenLaunch<true>{d_tiles,
+                ntiles, mmax, nmax, g, s});
+    return dispatch_kernel(g.ndim, g.family, GenLaunch<false>{d_tiles,
+            ntiles, mmax, nmax, g, s});
+}

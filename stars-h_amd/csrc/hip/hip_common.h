@@ -13,35 +13,6 @@
     return e_ == hipErrorOutOfMemory ? STARSH_MALLOC_ERROR : \
         STARSH_UNKNOWN_ERROR; } } while(0)
 
-// One far-field tile of a compression batch.  Offsets count doubles from
-// the base of the batch workspace; the device writes kc, p, rank, bnorm2,
-// tail2.
-struct TileDesc
-{
-    int m, n, k;       // tile rows, columns, sketch width
-    int gen;           // 1: A generated from points, 0: explicit A
-    int64_t r0, c0;    // first row / column point (cluster order)
-    int64_t a_off;     // explicit A (m x n, ld m) when gen == 0
-    int64_t om_off;    // test matrix Omega (n x k, ld n)
-    int64_t q_off;     // Y, then Q (m x k, ld m)
-    int64_t bt_off;    // Bt = A^T Q (n x k, ld n)
-    int64_t q2_off;    // Q2 (n x k, ld n)
-    int64_t r_off;     // R2 (k x k, ld k)
-    int64_t w_off;     // W_r (p x r, ld p), then J_r after it
-    int64_t s_off;     // singular values, k entries
-    int64_t coef_off;  // projection coefficients (k x 32, ld k)
-    int64_t u_off;     // U (m x maxrank, ld m)
-    int64_t v_off;     // V (n x maxrank, ld n)
-    int64_t rs_off;    // row scales of the int8 sketch (m)
-    int64_t om8_off;   // integer test matrix (bytes, n x k, ld n)
-    int kc;           // kept (numerically independent) sketch columns
-    int p;             // size of the core problem after truncation
-    int rank;          // chosen rank, -1 for a false far-field tile
-    int flags;
-    double bnorm2;     // |B|_F^2
-    double tail2;      // squared norm of the truncated rows of B
-};
-
 // Kernel parameters in the form the device evaluates them.  Derived on the
 // host with the same operations as the reference kernels.
 struct KernelParams
@@ -75,6 +46,42 @@ static inline KernelParams make_params(const starsh_hip_kernel_desc *kd)
     p.diag = kd->diag;
     return p;
 }
+
+// One far-field tile of a compression batch.  Offsets count doubles from
+// the base of the batch workspace; the device writes kc, p, rank, flags,
+// bnorm2, tail2.
+struct TileDesc
+{
+    int m, n, k;       // tile rows, columns, sketch width
+    int gen;           // 1: A generated from points, 0: explicit A
+    int64_t r0, c0;    // first row / column point (cluster order)
+    int64_t a_off;     // explicit A (m x n, ld m) when gen == 0
+    int64_t om_off;    // test matrix Omega (n x k, ld n)
+    int64_t q_off;     // Y, then Q (m x k, ld m)
+    int64_t bt_off;    // Bt = A^T Q (n x k, ld n)
+    int64_t q2_off;    // Q2 (n x k, ld n)
+    int64_t r_off;     // R2 (k x k, ld k)
+    int64_t w_off;     // W_r (p x r, ld p), then J_r after it
+    int64_t s_off;     // singular values, k entries
+    int64_t coef_off;  // projection coefficients (k x 32, ld k)
+    int64_t u_off;     // U (m x maxrank, ld m)
+    int64_t v_off;     // V (n x maxrank, ld n)
+    int64_t om8_off;   // integer test matrix (bytes, n x k, ld n)
+    int kc;           // kept (numerically independent) sketch columns
+    int p;             // size of the core problem after truncation
+    int rank;          // chosen rank, -1 for a false far-field tile
+    int flags;
+    double bnorm2;     // |B|_F^2
+    double tail2;      // squared norm of the truncated rows of B
+    // the tile's problem (a batch may mix problems of one kernel family
+    // and dimension)
+    const double *xr;  // row points in cluster order, SoA with stride nr
+    const double *xc;  // column points, SoA with stride nc
+    int64_t nr, nc;
+    KernelParams kp;
+    double tol;        // accuracy of the rank rule
+    int maxrank;
+};
 
 __device__ inline double matern_closed(double nu, double x)
 {
@@ -147,6 +154,66 @@ __device__ inline double kernel_value(const KernelParams &kp,
     return v;
 }
 
+// kernel_value with the family fixed at compile time and the row point
+// held in registers (x_i); same operations and order.
+template <int NDIM, int FAM>
+__device__ inline double kval(const KernelParams &kp, const double (&xi)[NDIM],
+        const double *__restrict__ yc, int64_t ny, int64_t j)
+{
+#pragma clang fp contract(off)
+    double dist = 0.0;
+#pragma unroll
+    for(int d = 0; d < NDIM; d++)
+    {
+        double t = xi[d]-yc[d*ny+j];
+        dist += t*t;
+    }
+    double v;
+    switch(FAM)
+    {
+        case STARSH_HIP_EXP:
+            dist = sqrt(dist)/kp.bexp;
+            v = dist == 0 ? kp.sigma+kp.noise : kp.sigma*exp(dist);
+            break;
+        case STARSH_HIP_SQREXP:
+            dist = dist/kp.bsqr;
+            v = dist == 0 ? kp.sigma+kp.noise : kp.sigma*exp(dist);
+            break;
+        case STARSH_HIP_MATERN:
+            dist = sqrt(dist)*kp.theta;
+            v = dist == 0 ? kp.sigma+kp.noise :
+                kp.sigma*matern_closed(kp.nu, dist);
+            break;
+        case STARSH_HIP_MATERN2:
+            dist = sqrt(dist)/kp.beta;
+            v = dist == 0 ? kp.sigma+kp.noise :
+                kp.sigma*matern_closed(kp.nu, dist);
+            break;
+        case STARSH_HIP_COULOMB:
+            v = dist == 0 ? 0.0 : 1.0/sqrt(dist);
+            break;
+        case STARSH_HIP_SIN:
+            if(dist == 0)
+                v = kp.diag;
+            else
+            {
+                dist = sqrt(dist);
+                v = sin(kp.wave_k*dist)/dist;
+            }
+            break;
+        default:
+            if(dist == 0)
+                v = kp.diag;
+            else
+            {
+                dist = sqrt(dist);
+                v = cos(kp.wave_k*dist)/dist;
+            }
+            break;
+    }
+    return v;
+}
+
 // Points of a batch: row and column point sets in cluster order, SoA.
 struct PointSets
 {
@@ -158,3 +225,34 @@ struct PointSets
 
 // Library stream (created by starsh_hip_init).
 extern "C" hipStream_t starsh_hip_stream(void);
+
+// Calls F.template operator()<NDIM, FAM>() for the runtime (ndim, family).
+template <typename F>
+inline int dispatch_kernel(int ndim, int family, F &&f)
+{
+#define STARSH_FAM_CASES(D) \
+    switch(family) \
+    { \
+        case STARSH_HIP_EXP: return f.template operator()<D, \
+            STARSH_HIP_EXP>(); \
+        case STARSH_HIP_SQREXP: return f.template operator()<D, \
+            STARSH_HIP_SQREXP>(); \
+        case STARSH_HIP_MATERN: return f.template operator()<D, \
+            STARSH_HIP_MATERN>(); \
+        case STARSH_HIP_MATERN2: return f.template operator()<D, \
+            STARSH_HIP_MATERN2>(); \
+        case STARSH_HIP_COULOMB: return f.template operator()<D, \
+            STARSH_HIP_COULOMB>(); \
+        case STARSH_HIP_SIN: return f.template operator()<D, \
+            STARSH_HIP_SIN>(); \
+        default: return f.template operator()<D, STARSH_HIP_COS>(); \
+    }
+    switch(ndim)
+    {
+        case 1: STARSH_FAM_CASES(1)
+        case 2: STARSH_FAM_CASES(2)
+        case 3: STARSH_FAM_CASES(3)
+        default: STARSH_FAM_CASES(4)
+    }
+#undef STARSH_FAM_CASES
+}

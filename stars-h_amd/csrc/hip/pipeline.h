@@ -5,9 +5,7 @@
 
 enum GemmMode
 {
-    GM_SKETCH_GEN = 0,
-    GM_SKETCH_MEM,
-    GM_PROJ_GEN,
+    GM_SKETCH_MEM = 0,
     GM_PROJ_MEM,
     GM_COEF,
     GM_UPDATE,
@@ -21,14 +19,18 @@ struct GemmArgs
     double *ws;           // batch workspace base
     double *out;          // output base for U / V
     const double *abase;  // explicit tiles base (dense mode)
-    KernelParams kp;
-    PointSets ps;
+    int ndim, family;     // kernel of every tile of the batch
     int j0, jb;           // panel position (COEF / UPDATE)
     int which;            // 0: Q (m rows), 1: Q2 (n rows)
 };
 
 int pipeline_gemm(int mode, const TileDesc *d_tiles, int ntiles, int mmax,
         int nmax, const GemmArgs &g, hipStream_t s);
+
+// Sketch (proj = 0) or projection (proj = 1) with the kernel tile generated
+// in registers (gen_gemm.hip).
+int pipeline_gen_gemm(int proj, const TileDesc *d_tiles, int ntiles,
+        int mmax, int nmax, const GemmArgs &g, hipStream_t s);
 
 // Column norms of the m x k buffers (which = 0) or of Bt (which = 2) into
 // d_norm (ntiles x kmax).
@@ -38,10 +40,16 @@ int pipeline_colnorm(const TileDesc *d_tiles, int ntiles, int which,
 // One block classical Gram-Schmidt panel pass with intra-panel CGS2 and
 // deflation.  which: 0 = Q (width k), 1 = Q2 (width p).  Columns whose
 // norm after projection is at most tau * ref (ref = d_ref[t*kmax+j], or 1
-// when d_ref is null) are set to zero and flagged in d_keep.
+// when d_ref is null) are set to zero and flagged in d_keep.  The first
+// pass (d_ref not null) marks a tile whose whole panel was dropped as done;
+// *d_active counts the tiles that still have panels to process.
 int pipeline_panel(const TileDesc *d_tiles, int ntiles, int which,
         double *ws, int j0, int jb, const double *d_ref, int kmax,
-        double tau, int *d_keep, int rows_max, hipStream_t s);
+        double tau, int *d_keep, int rows_max, int *d_active, hipStream_t s);
+
+// *d_active = number of tiles with a nonzero width (k or p).
+int pipeline_count_active(const TileDesc *d_tiles, int ntiles, int which,
+        int *d_active, hipStream_t s);
 
 // Moves kept columns of Q to the front; sets kc.
 int pipeline_compact(TileDesc *d_tiles, int ntiles, double *ws,
@@ -49,8 +57,9 @@ int pipeline_compact(TileDesc *d_tiles, int ntiles, double *ws,
 
 // From column norms of Bt: |B|_F^2, truncation p and tail; copies
 // Bt[:, :p] into the Q2 buffer.
+// The tail bound is (eta*tol)^2 |B|_F^2 with each tile's own tol.
 int pipeline_truncate(TileDesc *d_tiles, int ntiles, double *ws,
-        const double *d_bnorm, int kmax, double eta_tol, int nmax,
+        const double *d_bnorm, int kmax, double eta, int nmax,
         hipStream_t s);
 
 // Integer test matrix (n x k bytes) for the int8 sketch.
@@ -61,7 +70,7 @@ int pipeline_sketch_i8(const TileDesc *d_tiles, int ntiles, int mmax,
         int kmax, const GemmArgs &g, const signed char *d_om8,
         hipStream_t s);
 
-// One-sided Jacobi SVD of R2, singular values, rank by the reference rule,
-// W_r and J_r.
-int pipeline_core(TileDesc *d_tiles, int ntiles, double *ws, int maxrank,
-        double tol, int pmax, hipStream_t s);
+// One-sided Jacobi SVD of R2, singular values, rank by the reference rule
+// (each tile's own maxrank and tol), W_r and J_r.
+int pipeline_core(TileDesc *d_tiles, int ntiles, double *ws, int pmax,
+        hipStream_t s);

@@ -41,15 +41,51 @@ struct DeviceBuf
 #define TRY(expr) do { int i_ = (expr); if(i_ != STARSH_SUCCESS) return i_; } \
     while(0)
 
+// Host view of the device counter of tiles still in a BCGS2 run.  The
+// counter is copied to pinned memory after every panel; the host reads it
+// LAG panels later (so the stream always holds queued work) and stops
+// launching once no tile is left.
+constexpr int LAG = 3;
+constexpr int NSLOT = 4;
+
+struct ActiveProbe
+{
+    int *h = nullptr;          // pinned host slots
+    hipEvent_t ev[NSLOT] = {};
+    int init()
+    {
+        if(h)
+            return STARSH_SUCCESS;
+        HIP_TRY(hipHostMalloc((void **)&h, sizeof(int)*NSLOT,
+                    hipHostMallocDefault));
+        for(int e = 0; e < NSLOT; e++)
+            HIP_TRY(hipEventCreateWithFlags(&ev[e], hipEventDisableTiming));
+        return STARSH_SUCCESS;
+    }
+};
+
+ActiveProbe probe;
+
 int bcgs2(TileDesc *d_tiles, int ntiles, int which, double *ws,
         const double *d_ref, int kmax, int width_max, int rows_max,
-        int *d_keep, double tau_first, GemmArgs g, hipStream_t s)
+        int *d_keep, double tau_first, int *d_active, GemmArgs g,
+        hipStream_t s)
 {
-    const int pb = rows_max <= 2048 ? 16 : 8;
+    const int pb = rows_max <= 2048 ? 8 : 4;
     g.which = which;
     g.jb = pb;
-    for(int j0 = 0; j0 < width_max; j0 += pb)
+    TRY(probe.init());
+    TRY(pipeline_count_active(d_tiles, ntiles, which, d_active, s));
+    int panel = 0;
+    for(int j0 = 0; j0 < width_max; j0 += pb, panel++)
     {
+        if(panel >= LAG)
+        {
+            const int slot = (panel-LAG) % NSLOT;
+            HIP_TRY(hipEventSynchronize(probe.ev[slot]));
+            if(probe.h[slot] == 0)
+                break;
+        }
         g.j0 = j0;
         for(int pass = 0; pass < 2; pass++)
         {
@@ -62,8 +98,12 @@ int bcgs2(TileDesc *d_tiles, int ntiles, int which, double *ws,
             TRY(pipeline_panel(d_tiles, ntiles, which, ws, j0, pb,
                         pass == 0 ? d_ref : nullptr, kmax,
                         pass == 0 ? tau_first : TAU_SECOND, d_keep, rows_max,
-                        s));
+                        d_active, s));
         }
+        const int slot = panel % NSLOT;
+        HIP_TRY(hipMemcpyAsync(probe.h+slot, d_active, sizeof(int),
+                    hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipEventRecord(probe.ev[slot], s));
     }
     return STARSH_SUCCESS;
 }
@@ -104,6 +144,29 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     out->d_uv = nullptr;
     if(nt <= 0)
         return STARSH_SUCCESS;
+    // problems of the batch
+    RsddProblem single;
+    single.kd = in->kd;
+    single.d_xr = in->d_xr;
+    single.d_xc = in->d_xc;
+    single.nr = in->nr;
+    single.nc = in->nc;
+    single.maxrank = in->maxrank;
+    single.oversample = in->oversample;
+    single.tol = in->tol;
+    const RsddProblem *probs = in->probs ? in->probs : &single;
+    const int nprob = in->probs ? in->nprob : 1;
+    std::vector<KernelParams> kps(nprob);
+    for(int q = 0; q < nprob; q++)
+    {
+        memset(&kps[q], 0, sizeof(KernelParams));
+        if(probs[q].kd)
+            kps[q] = make_params(probs[q].kd);
+        else
+            kps[q].ndim = 1;
+        if(kps[q].ndim != kps[0].ndim || kps[q].family != kps[0].family)
+            return STARSH_WRONG_PARAMETER;
+    }
     std::vector<TileDesc> h(nt);
     std::map<std::pair<int, int>, int64_t> omegas;
     int64_t off = 0;
@@ -112,10 +175,21 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     {
         TileDesc &d = h[t];
         memset(&d, 0, sizeof(d));
+        const int q = in->probs && in->prob ? in->prob[t] : 0;
+        if(q < 0 || q >= nprob)
+            return STARSH_WRONG_PARAMETER;
+        const RsddProblem &pr = probs[q];
+        d.xr = pr.d_xr;
+        d.xc = pr.d_xc;
+        d.nr = pr.nr;
+        d.nc = pr.nc;
+        d.kp = kps[q];
+        d.tol = pr.tol;
+        d.maxrank = pr.maxrank;
         d.m = in->m[t];
         d.n = in->n[t];
         int mn = d.m < d.n ? d.m : d.n;
-        d.k = in->k ? in->k[t] : in->maxrank+in->oversample;
+        d.k = in->k ? in->k[t] : pr.maxrank+pr.oversample;
         if(d.k > mn)
             d.k = mn;
         if(d.k < 1)
@@ -157,7 +231,7 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     {
         TileDesc &d = h[t];
         int64_t m = d.m, n = d.n, k = d.k;
-        int64_t rmax = in->maxrank < k ? in->maxrank : k;
+        int64_t rmax = d.maxrank < k ? d.maxrank : k;
         d.q_off = off; off += m*k;
         d.bt_off = off; off += n*k;
         d.q2_off = off; off += n*k;
@@ -165,7 +239,6 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
         d.w_off = off; off += 2*k*k;
         d.s_off = off; off += k;
         d.coef_off = off; off += k*16;
-        d.rs_off = off; off += in->sketch_int8 ? m : 0;
         d.u_off = uvoff; uvoff += m*rmax;
         d.v_off = uvoff; uvoff += n*rmax;
         out->uoff[t] = d.u_off;
@@ -174,7 +247,7 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     if(in->rows_out)
         for(int t = 0; t < nt; t++)
             in->rows_out[t] = h[t].k;
-    DeviceBuf ws, tiles, keep, norms, uv, om8;
+    DeviceBuf ws, tiles, keep, norms, uv, om8, active;
     TRY(ws.alloc(sizeof(double)*off));
     if(in->sketch_int8)
         TRY(om8.alloc(off8));
@@ -182,6 +255,7 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     TRY(keep.alloc(sizeof(int)*(size_t)nt*kmax));
     TRY(norms.alloc(sizeof(double)*(size_t)nt*kmax));
     TRY(uv.alloc(sizeof(double)*uvoff));
+    TRY(active.alloc(sizeof(int)));
     TileDesc *d_tiles = (TileDesc *)tiles.p;
     double *d_ws = (double *)ws.p;
     HIP_TRY(hipMemcpyAsync(d_tiles, h.data(), sizeof(TileDesc)*nt,
@@ -201,11 +275,8 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     g.ws = d_ws;
     g.out = (double *)uv.p;
     g.abase = in->d_a;
-    if(in->kd)
-        g.kp = make_params(in->kd);
-    else
-        g.kp.ndim = 1;
-    g.ps = PointSets{in->d_xr, in->nr, in->d_xc, in->nc};
+    g.ndim = kps[0].ndim;
+    g.family = kps[0].family;
 
     hipEvent_t ev[4];
     for(int e = 0; e < 4; e++)
@@ -215,32 +286,41 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     HIP_TRY(hipEventRecord(ev[2], s));
     if(in->sketch_int8)
     {
-        if(in->d_a != nullptr || in->kd == nullptr)
+        if(in->d_a != nullptr || probs[0].kd == nullptr)
             return STARSH_WRONG_PARAMETER;
         TRY(pipeline_sketch_i8(d_tiles, nt, mmax, kmax, g,
                     (const signed char *)om8.p, s));
     }
     else
-        TRY(pipeline_gemm(in->d_a ? GM_SKETCH_MEM : GM_SKETCH_GEN, d_tiles,
-                    nt, mmax, kmax, g, s));
+    {
+        if(in->d_a)
+            TRY(pipeline_gemm(GM_SKETCH_MEM, d_tiles, nt, mmax, kmax, g, s));
+        else
+            TRY(pipeline_gen_gemm(0, d_tiles, nt, mmax, kmax, g, s));
+    }
     HIP_TRY(hipEventRecord(ev[3], s));
     dump("omega", d_ws+h[0].om_off, (int64_t)h[0].n*h[0].k, s);
     dump("Y", d_ws+h[0].q_off, (int64_t)h[0].m*h[0].k, s);
     // 2. orthonormal basis with deflation, kept columns to the front
     TRY(pipeline_colnorm(d_tiles, nt, 0, d_ws, (double *)norms.p, kmax, s));
+    // columns of panels that are never launched count as dropped
+    HIP_TRY(hipMemsetAsync(keep.p, 0, sizeof(int)*(size_t)nt*kmax, s));
     TRY(bcgs2(d_tiles, nt, 0, d_ws, (double *)norms.p, kmax, kmax, mmax,
-                (int *)keep.p, in->tau > 0 ? in->tau : TAU_FIRST, g, s));
+                (int *)keep.p, in->tau > 0 ? in->tau : TAU_FIRST,
+                (int *)active.p, g, s));
     dump("Qfull", d_ws+h[0].q_off, (int64_t)h[0].m*h[0].k, s);
     TRY(pipeline_compact(d_tiles, nt, d_ws, (int *)keep.p, kmax, s));
     dump("Q", d_ws+h[0].q_off, (int64_t)h[0].m*h[0].k, s);
     // 3. projection Bt = A^T Q
-    TRY(pipeline_gemm(in->d_a ? GM_PROJ_MEM : GM_PROJ_GEN, d_tiles, nt, nmax,
-                kmax, g, s));
+    if(in->d_a)
+        TRY(pipeline_gemm(GM_PROJ_MEM, d_tiles, nt, nmax, kmax, g, s));
+    else
+        TRY(pipeline_gen_gemm(1, d_tiles, nt, nmax, kmax, g, s));
     dump("Bt", d_ws+h[0].bt_off, (int64_t)h[0].n*h[0].k, s);
     // 4. truncation of B's rows
     TRY(pipeline_colnorm(d_tiles, nt, 2, d_ws, (double *)norms.p, kmax, s));
     TRY(pipeline_truncate(d_tiles, nt, d_ws, (double *)norms.p, kmax,
-                ETA*in->tol, nmax, s));
+                ETA, nmax, s));
     HIP_TRY(hipMemcpyAsync(h.data(), d_tiles, sizeof(TileDesc)*nt,
                 hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -251,13 +331,13 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     if(pmax > 0)
     {
         TRY(bcgs2(d_tiles, nt, 1, d_ws, (double *)norms.p, kmax, pmax, nmax,
-                    (int *)keep.p, TAU_FIRST, g, s));
+                    (int *)keep.p, TAU_FIRST, (int *)active.p, g, s));
         dump("Q2", d_ws+h[0].q2_off, (int64_t)h[0].n*h[0].p, s);
         TRY(pipeline_gemm(GM_R2, d_tiles, nt, pmax, pmax, g, s));
         dump("R2", d_ws+h[0].r_off, (int64_t)h[0].k*h[0].k, s);
     }
     // 6. core SVD and rank
-    TRY(pipeline_core(d_tiles, nt, d_ws, in->maxrank, in->tol, pmax, s));
+    TRY(pipeline_core(d_tiles, nt, d_ws, pmax, s));
     HIP_TRY(hipMemcpyAsync(h.data(), d_tiles, sizeof(TileDesc)*nt,
                 hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));

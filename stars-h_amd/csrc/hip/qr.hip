@@ -11,12 +11,13 @@
 // rounding, and leaving them out changes the singular values of B = Q^T A
 // only at the level of tau*|A|.  Once a whole panel is dropped, the range
 // of the sketch is exhausted and the remaining panels of that tile are
-// skipped.
+// skipped; a device counter of tiles still in progress lets the host stop
+// launching panels when it reaches zero.
 #include "pipeline.h"
 
 namespace {
 
-constexpr int PANEL_THREADS = 1024;
+constexpr int PANEL_THREADS = 256;
 constexpr int NWAVES = PANEL_THREADS/64;
 
 __device__ inline double wave_sum(double v)
@@ -48,61 +49,36 @@ __global__ __launch_bounds__(256) void colnorm_kernel(const TileDesc *tiles,
     }
 }
 
-// Reduces cnt per-thread values (cnt <= PB) over the block; result in res.
-template <int PB>
-__device__ inline void block_sum(double (&v)[PB], int cnt, double *red,
-        double *res)
-{
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-    for(int l = 0; l < PB; l++)
-        if(l < cnt)
-        {
-            double s = wave_sum(v[l]);
-            if(lane == 0)
-                red[wave*PB+l] = s;
-        }
-    __syncthreads();
-    if(threadIdx.x < cnt)
-    {
-        double s = 0.0;
-#pragma unroll
-        for(int w = 0; w < NWAVES; w++)
-            s += red[w*PB+threadIdx.x];
-        res[threadIdx.x] = s;
-    }
-    __syncthreads();
-#pragma unroll
-    for(int l = 0; l < PB; l++)
-        if(l < cnt)
-            v[l] = res[l];
-    __syncthreads();
-}
-
 // Intra-panel CGS2 with deflation.  The panel (rows x PB) lives in
-// registers: thread r holds rows r, r+1024, ... (RPT of them).
+// registers: thread r holds rows r, r+256, ... (RPT of them).  Each dot
+// product is reduced within the wave by shuffles; the four wave partials go
+// to LDS and every thread sums them itself after one barrier.  The partial
+// buffers alternate between consecutive reductions, so a buffer is only
+// rewritten after the barrier that follows its last read.  Blocks are small
+// so that several tiles share a CU and hide each other's reductions.
 template <int RPT, int PB>
 __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(
         TileDesc *tiles, int which, double *ws, int j0, const double *ref,
-        int kmax, double tau2, int *keep, int first_pass)
+        int kmax, double tau2, int *keep, int first_pass, int *active)
 {
-    __shared__ double red[NWAVES*PB];
-    __shared__ double res[PB];
+    __shared__ double red[2][NWAVES][PB];
     TileDesc &td = tiles[blockIdx.x];
     const bool q2 = which != 0;
+    const int done_bit = q2 ? 2 : 1;
     const int rows = q2 ? td.n : td.m;
     const int width = q2 ? td.p : td.k;
     if(j0 >= width)
         return;
     const int jb = width-j0 < PB ? width-j0 : PB;
     int *kp = keep+(int64_t)blockIdx.x*kmax+j0;
-    if(td.flags & (q2 ? 2 : 1))
+    if(td.flags & done_bit)
     {
-        // Range already exhausted: every remaining column is dropped.
+        // range already exhausted: every remaining column is dropped
         if(threadIdx.x < jb)
             kp[threadIdx.x] = 0;
         return;
     }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double *W = ws+(q2 ? td.q2_off : td.q_off)+(int64_t)j0*rows;
     double w[RPT][PB];
 #pragma unroll
@@ -114,6 +90,7 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(
             w[r][c] = (i < rows && c < jb) ? W[(int64_t)c*rows+i] : 0.0;
     }
     int ndropped = 0;
+    int rb = 0;   // partial buffer of the next reduction
 #pragma unroll
     for(int c = 0; c < PB; c++)
     {
@@ -124,34 +101,47 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(
             {
                 if(c > 0)
                 {
-                    double d[PB];
-#pragma unroll
-                    for(int l = 0; l < PB; l++)
-                    {
-                        d[l] = 0.0;
-                        if(l < c)
-#pragma unroll
-                            for(int r = 0; r < RPT; r++)
-                                d[l] += w[r][l]*w[r][c];
-                    }
-                    block_sum<PB>(d, c, red, res);
 #pragma unroll
                     for(int l = 0; l < PB; l++)
                         if(l < c)
+                        {
+                            double d = 0.0;
 #pragma unroll
                             for(int r = 0; r < RPT; r++)
-                                w[r][c] -= d[l]*w[r][l];
+                                d += w[r][l]*w[r][c];
+                            d = wave_sum(d);
+                            if(lane == 0)
+                                red[rb][wave][l] = d;
+                        }
+                    __syncthreads();
+#pragma unroll
+                    for(int l = 0; l < PB; l++)
+                        if(l < c)
+                        {
+                            double coef = 0.0;
+#pragma unroll
+                            for(int q = 0; q < NWAVES; q++)
+                                coef += red[rb][q][l];
+#pragma unroll
+                            for(int r = 0; r < RPT; r++)
+                                w[r][c] -= coef*w[r][l];
+                        }
+                    rb ^= 1;
                 }
             }
-            double nv[PB];
-#pragma unroll
-            for(int l = 0; l < PB; l++)
-                nv[l] = 0.0;
+            double d = 0.0;
 #pragma unroll
             for(int r = 0; r < RPT; r++)
-                nv[0] += w[r][c]*w[r][c];
-            block_sum<PB>(nv, 1, red, res);
-            const double n2 = nv[0];
+                d += w[r][c]*w[r][c];
+            d = wave_sum(d);
+            if(lane == 0)
+                red[rb][wave][0] = d;
+            __syncthreads();
+            double n2 = 0.0;
+#pragma unroll
+            for(int q = 0; q < NWAVES; q++)
+                n2 += red[rb][q][0];
+            rb ^= 1;
             const double r2 = ref ? ref[(int64_t)blockIdx.x*kmax+j0+c] : 1.0;
             const bool drop = !(n2 > tau2*r2) || n2 == 0.0;
             const double inv = drop ? 0.0 : 1.0/sqrt(n2);
@@ -179,8 +169,26 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(
                 if(c < jb)
                     W[(int64_t)c*rows+i] = w[r][c];
     }
-    if(first_pass && ndropped == jb && threadIdx.x == 0)
-        td.flags |= q2 ? 2 : 1;
+    if(q2 && first_pass && ndropped == jb)
+    {
+        // Q2 is used in place (no compaction): the columns of the panels
+        // that will be skipped are zeroed here, so Q2^T Bt ignores them.
+        double *rest = W+(int64_t)jb*rows;
+        const int64_t cnt = (int64_t)(width-j0-jb)*rows;
+        for(int64_t e = threadIdx.x; e < cnt; e += PANEL_THREADS)
+            rest[e] = 0.0;
+    }
+    if(threadIdx.x == 0)
+    {
+        const bool last = j0+jb >= width;
+        if(first_pass && ndropped == jb)
+        {
+            td.flags |= done_bit;
+            atomicSub(active, 1);
+        }
+        else if(!first_pass && last)
+            atomicSub(active, 1);   // finished all panels
+    }
 }
 
 // Moves kept columns of Q to the front, in order; sets kc.
@@ -211,7 +219,7 @@ __global__ __launch_bounds__(256) void compact_kernel(TileDesc *tiles,
 // |B|_F^2, truncation point p (smallest p >= 1 whose row tail of B is at
 // most (eta*tol)^2 |B|_F^2) and the tail; copies Bt[:, :p] into Q2.
 __global__ __launch_bounds__(256) void truncate_kernel(TileDesc *tiles,
-        double *ws, const double *bnorm2, int kmax, double eta_tol)
+        double *ws, const double *bnorm2, int kmax, double eta)
 {
     TileDesc &td = tiles[blockIdx.x];
     const double *nb = bnorm2+(int64_t)blockIdx.x*kmax;
@@ -221,6 +229,7 @@ __global__ __launch_bounds__(256) void truncate_kernel(TileDesc *tiles,
         double total = 0.0;
         for(int c = 0; c < td.kc; c++)
             total += nb[c];
+        const double eta_tol = eta*td.tol;
         double limit = eta_tol*eta_tol*total;
         int p = td.kc;
         double tail = 0.0;
@@ -248,6 +257,28 @@ __global__ __launch_bounds__(256) void truncate_kernel(TileDesc *tiles,
         dst[e] = src[e];
 }
 
+// Number of tiles that take part in a BCGS2 run (width > 0).
+__global__ void count_active_kernel(const TileDesc *tiles, int ntiles,
+        int which, int *active)
+{
+    int c = 0;
+    for(int t = threadIdx.x; t < ntiles; t += blockDim.x)
+        c += (which ? tiles[t].p : tiles[t].k) > 0;
+    for(int o = 32; o > 0; o >>= 1)
+        c += __shfl_xor(c, o, 64);
+    __shared__ int part[16];
+    if((threadIdx.x & 63) == 0)
+        part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if(threadIdx.x == 0)
+    {
+        int s = 0;
+        for(int q = 0; q < (int)blockDim.x/64; q++)
+            s += part[q];
+        *active = s;
+    }
+}
+
 } // namespace
 
 int pipeline_colnorm(const TileDesc *d_tiles, int ntiles, int which,
@@ -264,27 +295,36 @@ int pipeline_colnorm(const TileDesc *d_tiles, int ntiles, int which,
     return STARSH_SUCCESS;
 }
 
+int pipeline_count_active(const TileDesc *d_tiles, int ntiles, int which,
+        int *d_active, hipStream_t s)
+{
+    hipLaunchKernelGGL(count_active_kernel, dim3(1), dim3(1024), 0, s,
+            d_tiles, ntiles, which, d_active);
+    HIP_TRY(hipGetLastError());
+    return STARSH_SUCCESS;
+}
+
 int pipeline_panel(const TileDesc *d_tiles, int ntiles, int which,
         double *ws, int j0, int jb, const double *d_ref, int kmax,
-        double tau, int *d_keep, int rows_max, hipStream_t s)
+        double tau, int *d_keep, int rows_max, int *d_active, hipStream_t s)
 {
     if(ntiles <= 0)
         return STARSH_SUCCESS;
     TileDesc *tiles = const_cast<TileDesc *>(d_tiles);
     const int first = d_ref != nullptr;
     const double tau2 = tau*tau;
-    if(rows_max <= 1024 && jb == 16)
-        hipLaunchKernelGGL((panel_kernel<1, 16>), dim3(ntiles),
-                dim3(PANEL_THREADS), 0, s, tiles, which, ws, j0, d_ref, kmax,
-                tau2, d_keep, first);
-    else if(rows_max <= 2048 && jb == 16)
-        hipLaunchKernelGGL((panel_kernel<2, 16>), dim3(ntiles),
-                dim3(PANEL_THREADS), 0, s, tiles, which, ws, j0, d_ref, kmax,
-                tau2, d_keep, first);
-    else if(rows_max <= 4096 && jb == 8)
+    if(rows_max <= 1024 && jb == 8)
         hipLaunchKernelGGL((panel_kernel<4, 8>), dim3(ntiles),
                 dim3(PANEL_THREADS), 0, s, tiles, which, ws, j0, d_ref, kmax,
-                tau2, d_keep, first);
+                tau2, d_keep, first, d_active);
+    else if(rows_max <= 2048 && jb == 8)
+        hipLaunchKernelGGL((panel_kernel<8, 8>), dim3(ntiles),
+                dim3(PANEL_THREADS), 0, s, tiles, which, ws, j0, d_ref, kmax,
+                tau2, d_keep, first, d_active);
+    else if(rows_max <= 4096 && jb == 4)
+        hipLaunchKernelGGL((panel_kernel<16, 4>), dim3(ntiles),
+                dim3(PANEL_THREADS), 0, s, tiles, which, ws, j0, d_ref, kmax,
+                tau2, d_keep, first, d_active);
     else
         return STARSH_WRONG_PARAMETER;
     HIP_TRY(hipGetLastError());
@@ -303,14 +343,14 @@ int pipeline_compact(TileDesc *d_tiles, int ntiles, double *ws,
 }
 
 int pipeline_truncate(TileDesc *d_tiles, int ntiles, double *ws,
-        const double *d_bnorm, int kmax, double eta_tol, int nmax,
+        const double *d_bnorm, int kmax, double eta, int nmax,
         hipStream_t s)
 {
     (void)nmax;
     if(ntiles <= 0)
         return STARSH_SUCCESS;
     hipLaunchKernelGGL(truncate_kernel, dim3(ntiles), dim3(256), 0, s,
-            d_tiles, ws, d_bnorm, kmax, eta_tol);
+            d_tiles, ws, d_bnorm, kmax, eta);
     HIP_TRY(hipGetLastError());
     return STARSH_SUCCESS;
 }

@@ -1,16 +1,19 @@
 // Mixed-precision sketch Y = A Omega on int8 MFMA (v_mfma_i32_16x16x64_i8)
-// with exact integer accumulation.
+// with exact integer accumulation inside each K chunk of 64 columns.
 //
-// Each row of a tile is scaled by a power of two 2^e_i >= max_j |A_ij|, and
-// the scaled entries r = A_ij / 2^e_i in (-1, 1) are split into four signed
-// 7-bit digits (Ozaki splitting):
-//   r = d1 2^-6 + d2 2^-13 + d3 2^-20 + d4 2^-27 + O(2^-28),  |d| <= 64.
-// The test matrix is integer: Omega_ij = round(32 x_ij) clamped to
-// [-127, 127], x_ij the dlarnv normal stream of the reference.  Every digit
-// product sum over a tile row is exact in int32 (|sum| <= n * 64 * 127), so
-// Y carries only the 2^-28 relative representation error of A and no
-// accumulation error.  Entries are generated in FP64 from the points, as in
-// the plain form; only the products run in int8.
+// For every row i and chunk c of 64 columns the entries are scaled by a
+// power of two 2^e >= max_{j in c} |A_ij| and rounded to an integer
+// X = rint(A_ij 2^(27-e)), |X| <= 2^27, which is split into four balanced
+// base-128 digits (Ozaki splitting):
+//   X = d1 2^21 + d2 2^14 + d3 2^7 + d4,   |d| <= 64,
+// so A_ij = 2^(e-27) X up to 2^-28 of the chunk's largest entry.  The test
+// matrix is integer: Omega_ij = round(32 x_ij) clamped to [-127, 127], x_ij
+// the dlarnv normal stream of the reference.  Each digit plane's product
+// with Omega over one chunk is exact in int32 (|sum| <= 64 * 64 * 127 <
+// 2^19); the planes are recombined exactly as 2^14 (2^7 s1 + s2) +
+// (2^7 s3 + s4) and added to the FP64 row of Y with the chunk's scale.
+// Entries are generated in FP64 from the points, as in the plain form; only
+// the products run in int8.
 #include "pipeline.h"
 
 typedef int int4v __attribute__((ext_vector_type(4)));
@@ -21,39 +24,6 @@ constexpr int RB = 64;           // rows per block
 constexpr int CB = 64;           // sketch columns per block
 constexpr int KB = 64;           // K chunk (one MFMA depth)
 constexpr int APAD = 80;         // bytes per LDS row (64 + 16 padding)
-
-// Row scales: 2^e with 2^(e-1) <= max_j |A_ij| < 2^e (1 for a zero row).
-template <int NDIM>
-__global__ __launch_bounds__(256) void rowscale_kernel(const TileDesc *tiles,
-        KernelParams kp, PointSets ps, double *ws)
-{
-    const TileDesc t = tiles[blockIdx.y];
-    const int i = blockIdx.x*64+(threadIdx.x >> 2);
-    const int part = threadIdx.x & 3;
-    double mx = 0.0;
-    if(i < t.m)
-        for(int j = part; j < t.n; j += 4)
-        {
-            double v = fabs(kernel_value<NDIM>(kp, ps.xr, ps.nr, t.r0+i,
-                        ps.xc, ps.nc, t.c0+j));
-            mx = v > mx ? v : mx;
-        }
-    double o = __shfl_xor(mx, 1, 64);
-    mx = o > mx ? o : mx;
-    o = __shfl_xor(mx, 2, 64);
-    mx = o > mx ? o : mx;
-    if(i < t.m && part == 0)
-    {
-        int e = 0;
-        double s = 1.0;
-        if(mx > 0.0)
-        {
-            frexp(mx, &e);      // mx = f 2^e, f in [0.5, 1)
-            s = ldexp(1.0, e);
-        }
-        ws[t.rs_off+i] = s;
-    }
-}
 
 __constant__ uint64_t c_apow_i8[64];
 
@@ -85,12 +55,18 @@ __device__ inline int pack4(int a, int b, int c, int d)
         ((unsigned)(d & 255) << 24);
 }
 
-template <int NDIM>
+// One block: RB rows x CB sketch columns of one tile.  Generation: thread
+// t owns row t/4 and 16 consecutive columns (t%4)*16.. of each chunk, so a
+// row's chunk maximum is a reduction over four neighbouring lanes.  MFMA:
+// wave w owns rows 16w.. and all CB columns.
+template <int NDIM, int FAM, int CB>
 __global__ __launch_bounds__(256) void sketch_i8_kernel(const TileDesc *tiles,
-        KernelParams kp, PointSets ps, double *ws, const signed char *om8)
+        double *ws, const signed char *om8)
 {
+    constexpr int NF = CB/16;
     __shared__ __align__(16) signed char As[4][RB*APAD];
     __shared__ __align__(16) signed char Bs[CB*APAD];
+    __shared__ double csc[RB];
     const TileDesc t = tiles[blockIdx.y];
     const int nbm = (t.m+RB-1)/RB;
     const int bm = blockIdx.x % nbm, bn = blockIdx.x / nbm;
@@ -98,40 +74,57 @@ __global__ __launch_bounds__(256) void sketch_i8_kernel(const TileDesc *tiles,
         return;
     const int row0 = bm*RB, col0 = bn*CB;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // generation assignment: one row, 16 consecutive K per thread
-    const int gi = tid & 63, gk = (tid >> 6)*16;
+    const int gi = tid >> 2, gk = (tid & 3)*16;
     const bool row_ok = row0+gi < t.m;
-    const double inv_scale = row_ok ? 1.0/ws[t.rs_off+row0+gi] : 0.0;
+    double xi[NDIM];
+#pragma unroll
+    for(int d = 0; d < NDIM; d++)
+        xi[d] = row_ok ? t.xr[d*t.nr+t.r0+row0+gi] : 0.0;
     const signed char *om = om8+t.om8_off;
-    int4v acc[4][4];
+    double yacc[NF][4];
 #pragma unroll
-    for(int s = 0; s < 4; s++)
+    for(int c = 0; c < NF; c++)
 #pragma unroll
-        for(int c = 0; c < 4; c++)
-            acc[s][c] = int4v{0, 0, 0, 0};
+        for(int v = 0; v < 4; v++)
+            yacc[c][v] = 0.0;
     for(int k0 = 0; k0 < t.n; k0 += KB)
     {
-        // ---- A: FP64 entries -> four int8 digit planes
-        int dg[4][16];
+        // ---- A: FP64 entries of the chunk, scale, four int8 digit planes
+        double a[16];
+        double mx = 0.0;
 #pragma unroll
         for(int e = 0; e < 16; e++)
         {
             const int kk = k0+gk+e;
-            double r = 0.0;
-            if(row_ok && kk < t.n)
-                r = kernel_value<NDIM>(kp, ps.xr, ps.nr, t.r0+row0+gi,
-                        ps.xc, ps.nc, t.c0+kk)*inv_scale;
-            double x = r*64.0;
-            double d = rint(x);
-            dg[0][e] = (int)d;
-            x = (x-d)*128.0;
-            d = rint(x);
-            dg[1][e] = (int)d;
-            x = (x-d)*128.0;
-            d = rint(x);
-            dg[2][e] = (int)d;
-            x = (x-d)*128.0;
-            dg[3][e] = (int)rint(x);
+            a[e] = (row_ok && kk < t.n) ? kval<NDIM, FAM>(t.kp, xi, t.xc,
+                    t.nc, t.c0+kk) : 0.0;
+            const double f = fabs(a[e]);
+            mx = f > mx ? f : mx;
+        }
+        double o = __shfl_xor(mx, 1, 64);
+        mx = o > mx ? o : mx;
+        o = __shfl_xor(mx, 2, 64);
+        mx = o > mx ? o : mx;
+        int ex = 0;
+        if(mx > 0.0)
+            frexp(mx, &ex);     // mx = f 2^ex, f in [0.5, 1)
+        const double mul = mx > 0.0 ? ldexp(1.0, 27-ex) : 0.0;
+        if((tid & 3) == 0)
+            csc[gi] = mx > 0.0 ? ldexp(1.0, ex-27) : 0.0;
+        int dg[4][16];
+#pragma unroll
+        for(int e = 0; e < 16; e++)
+        {
+            int x = (int)rint(a[e]*mul);
+            const int d4 = ((x+64) & 127)-64;
+            x = (x-d4) >> 7;
+            const int d3 = ((x+64) & 127)-64;
+            x = (x-d3) >> 7;
+            const int d2 = ((x+64) & 127)-64;
+            dg[3][e] = d4;
+            dg[2][e] = d3;
+            dg[1][e] = d2;
+            dg[0][e] = (x-d2) >> 7;
         }
 #pragma unroll
         for(int s = 0; s < 4; s++)
@@ -144,8 +137,10 @@ __global__ __launch_bounds__(256) void sketch_i8_kernel(const TileDesc *tiles,
             *(int4v *)&As[s][gi*APAD+gk] = v;
         }
         // ---- B: Omega columns col0.., K rows k0.. (column-major, ld n)
+#pragma unroll
+        for(int h = 0; h < CB/64; h++)
         {
-            const int j = tid >> 2, kq = (tid & 3)*16;
+            const int j = (tid >> 2)+64*h, kq = (tid & 3)*16;
             int4v v = int4v{0, 0, 0, 0};
             if(col0+j < t.k)
             {
@@ -168,28 +163,42 @@ __global__ __launch_bounds__(256) void sketch_i8_kernel(const TileDesc *tiles,
             *(int4v *)&Bs[j*APAD+kq] = v;
         }
         __syncthreads();
-        // ---- MFMA: wave owns rows wave*16.., all 64 columns, 4 digits
+        // ---- MFMA over the chunk, then exact recombination into FP64.
+        // The i32 output of v_mfma_i32_16x16x64_i8 holds rows
+        // 4*(lane/16)+v, column lane%16.
         const int ar = wave*16+(lane & 15), kq = (lane >> 4)*16;
-        int4v b[4];
-#pragma unroll
-        for(int c = 0; c < 4; c++)
-            b[c] = *(const int4v *)&Bs[(c*16+(lane & 15))*APAD+kq];
+        int4v as[4];
 #pragma unroll
         for(int s = 0; s < 4; s++)
-        {
-            const int4v a = *(const int4v *)&As[s][ar*APAD+kq];
+            as[s] = *(const int4v *)&As[s][ar*APAD+kq];
+        double sc[4];
 #pragma unroll
-            for(int c = 0; c < 4; c++)
-                acc[s][c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[c],
-                        acc[s][c], 0, 0, 0);
+        for(int v = 0; v < 4; v++)
+            sc[v] = csc[wave*16+4*(lane >> 4)+v];
+#pragma unroll
+        for(int c = 0; c < NF; c++)
+        {
+            const int4v b = *(const int4v *)&Bs[(c*16+(lane & 15))*APAD+kq];
+            int4v acc[4];
+#pragma unroll
+            for(int s = 0; s < 4; s++)
+                acc[s] = __builtin_amdgcn_mfma_i32_16x16x64_i8(as[s], b,
+                        int4v{0, 0, 0, 0}, 0, 0, 0);
+#pragma unroll
+            for(int v = 0; v < 4; v++)
+            {
+                const int hi = acc[0][v]*128+acc[1][v];
+                const int lo = acc[2][v]*128+acc[3][v];
+                yacc[c][v] = fma((double)hi*16384.0+(double)lo, sc[v],
+                        yacc[c][v]);
+            }
         }
         __syncthreads();
     }
-    // ---- epilogue: Y = 2^e_i/32 * sum_s 2^(-6-7s) acc_s; the i32 output of
-    // v_mfma_i32_16x16x64_i8 holds rows 4*(lane/16)+v, column lane%16.
+    // ---- epilogue: Y = sum over chunks / 32 (Omega's integer scale)
     double *Y = ws+t.q_off;
 #pragma unroll
-    for(int c = 0; c < 4; c++)
+    for(int c = 0; c < NF; c++)
     {
         const int gj = col0+c*16+(lane & 15);
         if(gj >= t.k)
@@ -198,12 +207,8 @@ __global__ __launch_bounds__(256) void sketch_i8_kernel(const TileDesc *tiles,
         for(int v = 0; v < 4; v++)
         {
             const int gr = row0+wave*16+4*(lane >> 4)+v;
-            if(gr >= t.m)
-                continue;
-            double y = (double)acc[0][c][v]*0x1p-6+(double)acc[1][c][v]*
-                0x1p-13+(double)acc[2][c][v]*0x1p-20+(double)acc[3][c][v]*
-                0x1p-27;
-            Y[(int64_t)gj*t.m+gr] = y*ws[t.rs_off+gr]*(1.0/32.0);
+            if(gr < t.m)
+                Y[(int64_t)gj*t.m+gr] = yacc[c][v]*(1.0/32.0);
         }
     }
 }
@@ -233,28 +238,37 @@ int pipeline_omega_i8(signed char *d_out, int ncols, int k, hipStream_t s)
     return STARSH_SUCCESS;
 }
 
+namespace {
+
+struct SketchLaunch
+{
+    const TileDesc *tiles;
+    int ntiles, nbm, kmax;
+    const GemmArgs &g;
+    const signed char *om8;
+    hipStream_t s;
+    template <int NDIM, int FAM> int operator()() const
+    {
+        if(kmax <= 64)
+            hipLaunchKernelGGL((sketch_i8_kernel<NDIM, FAM, 64>),
+                    dim3(nbm, ntiles), dim3(256), 0, s, tiles, g.ws, om8);
+        else
+            hipLaunchKernelGGL((sketch_i8_kernel<NDIM, FAM, 128>),
+                    dim3(nbm*((kmax+127)/128), ntiles), dim3(256), 0, s,
+                    tiles, g.ws, om8);
+        HIP_TRY(hipGetLastError());
+        return STARSH_SUCCESS;
+    }
+};
+
+} // namespace
+
 int pipeline_sketch_i8(const TileDesc *d_tiles, int ntiles, int mmax,
         int kmax, const GemmArgs &g, const signed char *d_om8, hipStream_t s)
 {
     if(ntiles <= 0 || mmax <= 0 || kmax <= 0)
         return STARSH_SUCCESS;
-    dim3 gr((mmax+RB-1)/RB, ntiles);
-    dim3 gs(((mmax+RB-1)/RB)*((kmax+CB-1)/CB), ntiles);
-    switch(g.kp.ndim)
-    {
-#define SK_CASE(D) case D: \
-        hipLaunchKernelGGL(rowscale_kernel<D>, gr, dim3(256), 0, s, d_tiles, \
-                g.kp, g.ps, g.ws); \
-        hipLaunchKernelGGL(sketch_i8_kernel<D>, gs, dim3(256), 0, s, \
-                d_tiles, g.kp, g.ps, g.ws, d_om8); \
-        break;
-        SK_CASE(1)
-        SK_CASE(2)
-        SK_CASE(3)
-        default:
-        SK_CASE(4)
-#undef SK_CASE
-    }
-    HIP_TRY(hipGetLastError());
-    return STARSH_SUCCESS;
+    const int nbm = (mmax+RB-1)/RB;
+    return dispatch_kernel(g.ndim, g.family,
+            SketchLaunch{d_tiles, ntiles, nbm, kmax, g, d_om8, s});
 }

@@ -31,23 +31,50 @@ int hip_driver_finish(STARSH_blrm **matrix, STARSH_blrf *format,
         starsh_hip_stats *st);
 void hip_driver_set_stats(const starsh_hip_stats *stats);
 
+/* Several problems compressed together (the single-problem drivers use
+ * nprob = 1). */
+typedef struct hip_batch
+{
+    int nprob;
+    hip_problem *hp;        /* [nprob] */
+    STARSH_blrf **F;        /* [nprob] */
+    const int *maxrank;     /* [nprob] */
+    const double *tol;      /* [nprob] */
+    hip_factors *hf;        /* [nprob] */
+} hip_batch;
+
 /* Decides whether a tile's result is final: (rank, sketch width used, full
  * sketch width of the plain form, pipeline flags, oversample). */
 typedef int (*hip_accept_fn)(int rank, int k, int kfull, int flags,
         int oversample);
 
-/* Compresses the far-field tiles idx[0..ntiles) with sketch widths k[] in
+/* Compresses far-field tiles idx[t] of problems pidx[t] (t < ntiles, all of
+ * one kernel family and dimension) with sketch widths k[] in
  * workspace-sized chunks; accepted results (all when accept is NULL) go to
- * hf (rank and U/V copied to the host buffers). */
-int hip_driver_run(const hip_problem *hp, STARSH_blrf *F, int maxrank,
-        double tol, STARSH_int ntiles, const STARSH_int *idx, const int *k,
-        int int8, double tau, hip_factors *hf, int *rank_out, int *flags_out,
-        int *accepted, hip_accept_fn accept, starsh_hip_stats *st);
-void hip_driver_rank_stats(const hip_factors *hf, STARSH_int nfar,
+ * B->hf[pidx[t]] (rank and U/V copied to the host buffers). */
+int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
+        const STARSH_int *idx, const int *k, int int8, double tau,
+        int *rank_out, int *flags_out, int *accepted, hip_accept_fn accept,
         starsh_hip_stats *st);
+void hip_driver_rank_stats(const hip_batch *B, starsh_hip_stats *st);
+
+/* The plain and mixed forms over a list of tiles of a batch. */
+int hip_plain_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
+        const STARSH_int *idx, starsh_hip_stats *st);
+int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
+        const STARSH_int *idx, starsh_hip_stats *st);
+
+/* Whole compression of nprob problems (plain form when mixed == 0):
+ * prepare, compress every far-field tile, finish each problem. */
+int hip_compress(int mixed, int nprob, STARSH_blrm **matrix,
+        STARSH_blrf **format, const int *maxrank, const double *tol,
+        int onfly);
 
 /* First rank bucket (index into 32, 64, 128, 256, 512) of the mixed form. */
 int hip_mixed_set_first_bucket(int b);
+
+/* Column block of the generated-operand GEMMs (0 = by size, 64/128/256). */
+int starsh_hip_gen_gemm_bn(int bn);
 size_t hip_driver_workspace_budget(void);
 double hip_driver_now(void);
 

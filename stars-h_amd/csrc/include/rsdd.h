@@ -10,10 +10,26 @@
 extern "C" {
 #endif
 
+/* One problem of a batch: its kernel, points and accuracy. */
+typedef struct rsdd_problem
+{
+    const starsh_hip_kernel_desc *kd;
+    const double *d_xr, *d_xc;   /* points in cluster order (device, SoA) */
+    int64_t nr, nc;
+    int maxrank, oversample;
+    double tol;
+} RsddProblem;
+
 typedef struct rsdd_input
 {
     int ntiles;
     const int *m, *n;            /* tile sizes */
+    /* Several problems (all of one kernel family and dimension): tile t
+     * belongs to probs[prob[t]].  When probs is NULL the single problem is
+     * given by the fields kd ... tol below. */
+    int nprob;
+    const RsddProblem *probs;
+    const int *prob;
     /* tile contents: either generated ... */
     const starsh_hip_kernel_desc *kd;
     const double *d_xr, *d_xc;   /* points in cluster order (device, SoA) */

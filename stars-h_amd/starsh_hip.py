@@ -180,6 +180,23 @@ def srand(seed):
     C.CDLL(None).srand(C.c_uint(seed))
 
 
+def approximate_batch(problems, maxrank, tol, onfly=0):
+    """starsh_blrm_approximate_batch over TLRProblem objects; each
+    problem's matrix is stored in its .M.  maxrank / tol: scalars or one
+    value per problem."""
+    n = len(problems)
+    mr = list(maxrank) if hasattr(maxrank, "__len__") else [maxrank] * n
+    tl = list(tol) if hasattr(tol, "__len__") else [tol] * n
+    Ms = (C.POINTER(BLRM) * n)()
+    Fs = (C.POINTER(BLRF) * n)(*[p.F for p in problems])
+    info = lib().starsh_blrm_approximate_batch(
+        Ms, Fs, C.c_int(n), (C.c_int * n)(*mr), (C.c_double * n)(*tl),
+        C.c_int(onfly))
+    for i, p in enumerate(problems):
+        p.M = Ms[i] if info == 0 else None
+    return info
+
+
 class TLRProblem:
     """A problem built through the C API, ready for compression.
 
@@ -329,6 +346,17 @@ class TLRProblem:
         self.M = C.POINTER(BLRM)()
         info = fn(C.byref(self.M), self.F, maxrank, tol, onfly)
         return info
+
+    def autotune(self, maxrank, tol, onfly=0):
+        """Runs starsh_hip_autotune; returns (first bucket, seconds per
+        candidate tried)."""
+        first = C.c_int(-1)
+        secs = (C.c_double * 5)(*([-1.0] * 5))
+        info = self.L.starsh_hip_autotune(self.F, C.c_int(maxrank),
+                                          C.c_double(tol), C.c_int(onfly),
+                                          C.byref(first), secs)
+        assert info == 0, info
+        return first.value, [x for x in secs if x >= 0]
 
     def ranks(self):
         M = self.M.contents

@@ -37,7 +37,7 @@ def test_int8_sketch_matches_exact_product(tmp_path):
         assert p.approximate(200, 1e-7, 0, fn) == 0
     finally:
         del os.environ["STARSH_HIP_DEBUG_DUMP"]
-        L.starsh_hip_set_option(b"mixed_first_bucket", C.c_double(0))
+        L.starsh_hip_set_option(b"mixed_first_bucket", C.c_double(-1))
     Y = np.fromfile(tmp_path / "Y.bin")
     m = 800
     k = Y.size // m
