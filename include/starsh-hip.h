@@ -97,7 +97,10 @@ int starsh_hip_get_stats(starsh_hip_stats *stats);
  *                         {32, 64, 128, 256, 512}, or -1 (default) for the
  *                         autotuner's schedule table (0 without an entry);
  *   "gemm_bn"             column block of the generated-tile GEMMs: 0 (by
- *                         size, default), 64, 128 or 256.
+ *                         size, default), 64, 128 or 256;
+ *   "release_workspace"   (value ignored) frees the device workspace and
+ *                         the pinned staging buffer, which are otherwise
+ *                         kept and reused between calls.
  * Returns STARSH_WRONG_PARAMETER for an unknown name or value. */
 int starsh_hip_set_option(const char *name, double value);
 

@@ -64,7 +64,6 @@ int starsh_hip_dense_rsdd(int ntiles, const int *m, const int *n,
             info = starsh_hip_memcpy_d2h(V+voff[t], out.d_uv+vo[t],
                     sizeof(double)*(size_t)n[t]*rank[t]);
     }
-    starsh_hip_free(out.d_uv);
     free(pp); free(kc); free(fl); free(uo); free(vo);
     return info;
 }

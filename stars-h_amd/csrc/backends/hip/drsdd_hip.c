@@ -49,6 +49,11 @@ int starsh_hip_set_option(const char *name, double value)
         return hip_mixed_set_first_bucket((int)value);
     if(!strcmp(name, "gemm_bn"))
         return starsh_hip_gen_gemm_bn((int)value);
+    if(!strcmp(name, "release_workspace"))
+    {
+        rsdd_release_pools();
+        return STARSH_SUCCESS;
+    }
     return STARSH_WRONG_PARAMETER;
 }
 
@@ -415,7 +420,10 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
     int64_t *uo = malloc(sizeof(int64_t)*ntiles);
     int64_t *vo = malloc(sizeof(int64_t)*ntiles);
     RsddProblem *probs = malloc(sizeof(RsddProblem)*B->nprob);
-    if(!m || !n || !pp || !kc || !r0 || !c0 || !uo || !vo || !probs)
+    double **du = malloc(sizeof(double *)*ntiles);
+    double **dv = malloc(sizeof(double *)*ntiles);
+    if(!m || !n || !pp || !kc || !r0 || !c0 || !uo || !vo || !probs || !du ||
+            !dv)
         info = STARSH_MALLOC_ERROR;
     for(int q = 0; info == STARSH_SUCCESS && q < B->nprob; q++)
     {
@@ -491,7 +499,7 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
         st->dominant_kernel_seconds += out.sketch_seconds;
         st->dominant_kernel_launches += 1;
         double tc = hip_driver_now();
-        for(STARSH_int t = b0; t < b1 && info == STARSH_SUCCESS; t++)
+        for(STARSH_int t = b0; t < b1; t++)
         {
             const int q = pidx[t];
             const int mn = m[t] < n[t] ? m[t] : n[t];
@@ -500,6 +508,7 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
             kfull = kfull < mn ? kfull : mn;
             accepted[t] = accept ? accept(rank_out[t], kk, kfull,
                     flags_out[t], B->hp[q].oversample) : 1;
+            du[t] = dv[t] = NULL;
             if(!accepted[t])
                 continue;
             hip_factors *hf = B->hf+q;
@@ -507,19 +516,16 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
             hf->far_rank[bi] = rank_out[t];
             if(rank_out[t] <= 0)
                 continue;
-            info = starsh_hip_memcpy_d2h(hf->alloc_U+hf->uoff[bi],
-                    out.d_uv+uo[t], sizeof(double)*(size_t)m[t]*rank_out[t]);
-            if(info == STARSH_SUCCESS)
-                info = starsh_hip_memcpy_d2h(hf->alloc_V+hf->voff[bi],
-                        out.d_uv+vo[t], sizeof(double)*(size_t)n[t]*
-                        rank_out[t]);
+            du[t] = hf->alloc_U+hf->uoff[bi];
+            dv[t] = hf->alloc_V+hf->voff[bi];
         }
+        info = rsdd_copy_factors(&out, (int)(b1-b0), m+b0, n+b0, du+b0,
+                dv+b0);
         st->copy_seconds += hip_driver_now()-tc;
-        starsh_hip_free(out.d_uv);
         b0 = b1;
     }
     free(m); free(n); free(pp); free(kc); free(r0); free(c0); free(uo);
-    free(vo); free(probs);
+    free(vo); free(probs); free(du); free(dv);
     return info;
 }
 

@@ -6,7 +6,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <algorithm>
 #include <map>
+#include <thread>
 #include <vector>
 #include <utility>
 #include "pipeline.h"
@@ -40,6 +42,50 @@ struct DeviceBuf
 
 #define TRY(expr) do { int i_ = (expr); if(i_ != STARSH_SUCCESS) return i_; } \
     while(0)
+
+// Grow-only device (or pinned host) buffer kept between calls, so that
+// repeated compressions do not pay for allocating tens of GB each time.
+struct Pool
+{
+    void *p = nullptr;
+    size_t bytes = 0;
+    bool host = false;
+    int reserve(size_t want)
+    {
+        if(want <= bytes && p)
+            return STARSH_SUCCESS;
+        release();
+        // round up to 64 MiB to absorb small variations between calls
+        const size_t unit = (size_t)64 << 20;
+        size_t b = (want+unit-1)/unit*unit;
+        hipError_t e = host ? hipHostMalloc(&p, b, hipHostMallocDefault) :
+            hipMalloc(&p, b);
+        if(e != hipSuccess)
+        {
+            p = nullptr;
+            fprintf(stderr, "STARSH HIP ERROR: allocation of %zu bytes "
+                    "failed\n", b);
+            return STARSH_MALLOC_ERROR;
+        }
+        bytes = b;
+        return STARSH_SUCCESS;
+    }
+    void release()
+    {
+        if(p)
+        {
+            (void)hipStreamSynchronize(starsh_hip_stream());
+            if(host)
+                (void)hipHostFree(p);
+            else
+                (void)hipFree(p);
+        }
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+Pool ws_pool, uv_pool, pack_pool, stage_pool{nullptr, 0, true};
 
 // Host view of the device counter of tiles still in a BCGS2 run.  The
 // counter is copied to pinned memory after every panel; the host reads it
@@ -164,9 +210,11 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
             kps[q] = make_params(probs[q].kd);
         else
             kps[q].ndim = 1;
-        if(kps[q].ndim != kps[0].ndim || kps[q].family != kps[0].family)
-            return STARSH_WRONG_PARAMETER;
     }
+    // the tiles of one batch share a kernel family and dimension
+    const int q0 = in->probs && in->prob ? in->prob[0] : 0;
+    if(q0 < 0 || q0 >= nprob)
+        return STARSH_WRONG_PARAMETER;
     std::vector<TileDesc> h(nt);
     std::map<std::pair<int, int>, int64_t> omegas;
     int64_t off = 0;
@@ -176,7 +224,8 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
         TileDesc &d = h[t];
         memset(&d, 0, sizeof(d));
         const int q = in->probs && in->prob ? in->prob[t] : 0;
-        if(q < 0 || q >= nprob)
+        if(q < 0 || q >= nprob || kps[q].ndim != kps[q0].ndim ||
+                kps[q].family != kps[q0].family)
             return STARSH_WRONG_PARAMETER;
         const RsddProblem &pr = probs[q];
         d.xr = pr.d_xr;
@@ -247,17 +296,17 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     if(in->rows_out)
         for(int t = 0; t < nt; t++)
             in->rows_out[t] = h[t].k;
-    DeviceBuf ws, tiles, keep, norms, uv, om8, active;
-    TRY(ws.alloc(sizeof(double)*off));
+    DeviceBuf tiles, keep, norms, om8, active;
+    TRY(ws_pool.reserve(sizeof(double)*off));
     if(in->sketch_int8)
         TRY(om8.alloc(off8));
     TRY(tiles.alloc(sizeof(TileDesc)*nt));
     TRY(keep.alloc(sizeof(int)*(size_t)nt*kmax));
     TRY(norms.alloc(sizeof(double)*(size_t)nt*kmax));
-    TRY(uv.alloc(sizeof(double)*uvoff));
+    TRY(uv_pool.reserve(sizeof(double)*(uvoff > 0 ? uvoff : 1)));
     TRY(active.alloc(sizeof(int)));
     TileDesc *d_tiles = (TileDesc *)tiles.p;
-    double *d_ws = (double *)ws.p;
+    double *d_ws = (double *)ws_pool.p;
     HIP_TRY(hipMemcpyAsync(d_tiles, h.data(), sizeof(TileDesc)*nt,
                 hipMemcpyHostToDevice, s));
     for(auto &o : omegas)
@@ -273,10 +322,10 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     GemmArgs g;
     memset(&g, 0, sizeof(g));
     g.ws = d_ws;
-    g.out = (double *)uv.p;
+    g.out = (double *)uv_pool.p;
     g.abase = in->d_a;
-    g.ndim = kps[0].ndim;
-    g.family = kps[0].family;
+    g.ndim = kps[q0].ndim;
+    g.family = kps[q0].family;
 
     hipEvent_t ev[4];
     for(int e = 0; e < 4; e++)
@@ -286,7 +335,7 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     HIP_TRY(hipEventRecord(ev[2], s));
     if(in->sketch_int8)
     {
-        if(in->d_a != nullptr || probs[0].kd == nullptr)
+        if(in->d_a != nullptr || probs[q0].kd == nullptr)
             return STARSH_WRONG_PARAMETER;
         TRY(pipeline_sketch_i8(d_tiles, nt, mmax, kmax, g,
                     (const signed char *)om8.p, s));
@@ -370,7 +419,101 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
         out->kc[t] = h[t].kc;
         out->flags[t] = h[t].flags;
     }
-    out->d_uv = (double *)uv.p;
-    uv.p = nullptr;
+    out->d_uv = (double *)uv_pool.p;
     return STARSH_SUCCESS;
+}
+
+namespace {
+
+// Gathers segments of the factor buffer into one packed device buffer.
+__global__ __launch_bounds__(256) void pack_kernel(const double *src,
+        double *dst, const int64_t *seg, int nseg)
+{
+    // seg[3 s .. 3 s + 2] = source offset, packed offset, length
+    const int s = blockIdx.y;
+    if(s >= nseg)
+        return;
+    const int64_t so = seg[3*s], po = seg[3*s+1], len = seg[3*s+2];
+    for(int64_t e = (int64_t)blockIdx.x*blockDim.x+threadIdx.x; e < len;
+            e += (int64_t)gridDim.x*blockDim.x)
+        dst[po+e] = src[so+e];
+}
+
+} // namespace
+
+// Copies the factors of the tiles with dst_u[t] != NULL to host memory:
+// U_t (m[t] x rank[t]) to dst_u[t], V_t (n[t] x rank[t]) to dst_v[t].  The
+// segments are packed on the device, brought over in one copy through
+// pinned memory, and scattered on the host by several threads.
+int rsdd_copy_factors(const RsddOutput *out, int ntiles, const int *m,
+        const int *n, double *const *dst_u, double *const *dst_v)
+{
+    hipStream_t s = starsh_hip_stream();
+    std::vector<int64_t> seg;
+    std::vector<double *> dst;
+    int64_t total = 0;
+    for(int t = 0; t < ntiles; t++)
+    {
+        if(dst_u[t] == nullptr || out->rank[t] <= 0)
+            continue;
+        const int64_t lu = (int64_t)m[t]*out->rank[t];
+        const int64_t lv = (int64_t)n[t]*out->rank[t];
+        seg.insert(seg.end(), {out->uoff[t], total, lu});
+        dst.push_back(dst_u[t]);
+        total += lu;
+        seg.insert(seg.end(), {out->voff[t], total, lv});
+        dst.push_back(dst_v[t]);
+        total += lv;
+    }
+    const int nseg = (int)dst.size();
+    if(nseg == 0)
+        return STARSH_SUCCESS;
+    TRY(pack_pool.reserve(sizeof(double)*total+sizeof(int64_t)*seg.size()));
+    TRY(stage_pool.reserve(sizeof(double)*total));
+    double *d_pack = (double *)pack_pool.p;
+    int64_t *d_seg = (int64_t *)(d_pack+total);
+    HIP_TRY(hipMemcpyAsync(d_seg, seg.data(), sizeof(int64_t)*seg.size(),
+                hipMemcpyHostToDevice, s));
+    for(int s0 = 0; s0 < nseg; s0 += 65535)
+    {
+        const int cnt = std::min(nseg-s0, 65535);
+        hipLaunchKernelGGL(pack_kernel, dim3(16, cnt), dim3(256), 0, s,
+                out->d_uv, d_pack, d_seg+3*(int64_t)s0, cnt);
+        HIP_TRY(hipGetLastError());
+    }
+    double *h = (double *)stage_pool.p;
+    HIP_TRY(hipMemcpyAsync(h, d_pack, sizeof(double)*total,
+                hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    // host scatter: segments split over threads by size
+    unsigned nth = std::thread::hardware_concurrency();
+    nth = nth < 1 ? 1 : (nth > 16 ? 16 : nth);
+    if(total < ((int64_t)1 << 20))
+        nth = 1;
+    auto work = [&](unsigned id) {
+        const int64_t lo = total*id/nth, hi = total*(id+1)/nth;
+        for(int q = 0; q < nseg; q++)
+        {
+            const int64_t po = seg[3*q+1], len = seg[3*q+2];
+            const int64_t a = std::max(lo, po), b = std::min(hi, po+len);
+            if(a < b)
+                memcpy(dst[q]+(a-po), h+a, sizeof(double)*(b-a));
+        }
+    };
+    std::vector<std::thread> th;
+    for(unsigned id = 1; id < nth; id++)
+        th.emplace_back(work, id);
+    work(0);
+    for(auto &x : th)
+        x.join();
+    return STARSH_SUCCESS;
+}
+
+// Frees the buffers kept between calls.
+void rsdd_release_pools(void)
+{
+    ws_pool.release();
+    uv_pool.release();
+    pack_pool.release();
+    stage_pool.release();
 }

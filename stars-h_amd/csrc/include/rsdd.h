@@ -55,13 +55,23 @@ typedef struct rsdd_output
     int *rank;                   /* ntiles: rank, -1 for a false far tile */
     int *p, *kc, *flags;         /* ntiles: diagnostics */
     int64_t *uoff, *voff;        /* ntiles: offsets of U, V in d_uv */
-    double *d_uv;                /* device buffer with U and V (caller frees
-                                    with starsh_hip_free) */
+    double *d_uv;                /* device buffer with U and V, owned by
+                                    the pipeline and valid until its next
+                                    call */
     double device_seconds;
     double sketch_seconds;
 } RsddOutput;
 
 int rsdd_batch(const RsddInput *in, RsddOutput *out);
+
+/* Copies U_t (m[t] x rank[t], ld m[t]) to dst_u[t] and V_t (n[t] x rank[t])
+ * to dst_v[t] for every tile t < ntiles of the last rsdd_batch with
+ * dst_u[t] != NULL and rank > 0, in one device-to-host transfer. */
+int rsdd_copy_factors(const RsddOutput *out, int ntiles, const int *m,
+        const int *n, double *const *dst_u, double *const *dst_v);
+
+/* Frees the workspace and staging buffers kept between calls. */
+void rsdd_release_pools(void);
 
 /* Per tile |A|_F^2 and |A - U V^T|_F^2 with A regenerated on the GPU. */
 int starsh_hip_tile_errors(const starsh_hip_kernel_desc *kd,

@@ -180,6 +180,23 @@ def srand(seed):
     C.CDLL(None).srand(C.c_uint(seed))
 
 
+def spheres(nspheres, npoints, radius, seed):
+    """Points of starsh_particles_generate_spheres after srand(seed), as a
+    3 x N array (SoA)."""
+    L = lib()
+    srand(seed)
+    data = C.POINTER(Particles)()
+    assert L.starsh_particles_generate_spheres(
+        C.byref(data), STARSH_int(nspheres), STARSH_int(npoints),
+        C.c_double(radius)) == 0
+    part = data.contents
+    n = part.count * part.ndim
+    pts = np.ctypeslib.as_array(part.point, shape=(n,)).reshape(
+        part.ndim, part.count).copy()
+    L.starsh_particles_free(data)
+    return pts
+
+
 def approximate_batch(problems, maxrank, tol, onfly=0):
     """starsh_blrm_approximate_batch over TLRProblem objects; each
     problem's matrix is stored in its .M.  maxrank / tol: scalars or one
