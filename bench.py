@@ -13,6 +13,7 @@ reference's OpenMP RSDD) timed on a bounded sample of tiles on the host
 cores and scaled to the 820 tiles.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -99,6 +100,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="plain,mixed")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--first-bucket", type=int, default=None,
+                    help="force the mixed form's first rank bucket (index)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the compression path runs on a "
@@ -106,6 +109,9 @@ def main():
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
     import starsh_hip as S
+    if args.first_bucket is not None:
+        assert S.lib().starsh_hip_set_option(
+            b"mixed_first_bucket", ctypes.c_double(args.first_bucket)) == 0
     forms = {}
     for form in args.forms.split(","):
         backend = {"plain": "HIP", "mixed": "HIP_MIXED"}[form]

@@ -1,15 +1,16 @@
 // Core of the randomized SVD: singular values and vectors of the small
 // matrix R2 (p x p, with B[:p, :]^T = Q2 R2), the rank by the reference rule
-// (src/backends/sequential/dense/dsvfr.c:16-47) and the factors W_r and
-// J_r from which U and V are formed.
+// (src/backends/sequential/dense/dsvfr.c:16-47) and the matrix X from which
+// U and V are formed.
 //
-// One-sided (Hestenes) Jacobi on the columns of R2 gives R2 J = W with
-// orthogonal columns; the column norms of W are the singular values, J holds
-// the left singular vectors of B, and Q2 W holds the right ones scaled by
-// the singular values:
-//   U = Q J[:, :r],  V = Q2 W[:, :r]   (V = S Vs^T as in drsdd.c:83-87).
-// J[:, :r] is recovered as R2^T W[:, :r] diag(sigma)^-2 instead of being
-// accumulated through the sweeps.
+// One-sided (Hestenes) Jacobi on the columns of R2^T gives R2^T J = W with
+// orthogonal columns; the column norms of W are the singular values of B
+// and X = W diag(sigma)^-1 holds B's left singular vectors.  Hence, with
+// B = Us S Vs^T,
+//   U = Q[:, :p] X[:, :r],   V = Vs S = B^T Us = Bt[:, :p] X[:, :r]
+// (V = S Vs^T rows as in drsdd.c:83-87), and no rotation is accumulated.
+// R2 comes from a QR of B's leading rows, which are graded like the
+// singular values; Jacobi on R2^T converges in fewer sweeps than on R2.
 #include <float.h>
 #include "pipeline.h"
 
@@ -25,15 +26,21 @@ __device__ inline double group_sum(double v, int g)
     return v;
 }
 
-// W: p x p working matrix (ld = ldw), in LDS or global memory.
-__device__ void jacobi(double *W, int ldw, int p, int *sweeps_out)
+// W: p x p working matrix (ld = ldw), in LDS or global memory.  A group
+// of g threads handles one column pair, g consecutive rows at a time.  In
+// LDS the groups are as small as needed to give every pair of a round its
+// own group; in global memory a whole wave takes a pair, so that each
+// access of a column is one coalesced 512-byte segment.
+__device__ void jacobi(double *W, int ldw, int p, bool in_lds,
+        int *sweeps_out)
 {
     __shared__ int rotations;
     const int pe = p+(p & 1);
     const int npairs = pe/2;
     int g = 64;
-    while(g > 1 && g*npairs > CORE_THREADS)
-        g >>= 1;
+    if(in_lds)
+        while(g > 1 && g*npairs > CORE_THREADS)
+            g >>= 1;
     const int ngroups = CORE_THREADS/g;
     const int grp = threadIdx.x/g, gl = threadIdx.x % g;
     const double ctol = (double)p*DBL_EPSILON;
@@ -136,14 +143,15 @@ __global__ __launch_bounds__(CORE_THREADS) void core_kernel(TileDesc *tiles,
         W = ws+td.r_off+(int64_t)k*k;
         ldw = p;
     }
+    // W = R2^T
     for(int e = threadIdx.x; e < p*p; e += blockDim.x)
     {
         int i = e % p, j = e / p;
-        W[(int64_t)j*ldw+i] = R2[(int64_t)j*k+i];
+        W[(int64_t)j*ldw+i] = R2[(int64_t)i*k+j];
     }
     __syncthreads();
     int sweeps;
-    jacobi(W, ldw, p, &sweeps);
+    jacobi(W, ldw, p, p <= lds_p, &sweeps);
     __syncthreads();
     // Singular values: column norms of W.
     for(int j = threadIdx.x; j < p; j += blockDim.x)
@@ -209,24 +217,12 @@ __global__ __launch_bounds__(CORE_THREADS) void core_kernel(TileDesc *tiles,
     const int r = s_rank;
     if(r <= 0)
         return;
-    // W_r (p x r) then J_r = R2^T W_r diag(sigma)^-2 (p x r).
-    double *Wr = ws+td.w_off, *Jr = Wr+(int64_t)p*r;
+    // X (p x r): the r leading columns of W scaled to unit length
+    double *X = ws+td.w_off;
     for(int e = threadIdx.x; e < p*r; e += blockDim.x)
     {
         int i = e % p, c = e / p;
-        Wr[e] = W[(int64_t)perm[c]*ldw+i];
-    }
-    __syncthreads();
-    for(int e = threadIdx.x; e < p*r; e += blockDim.x)
-    {
-        int i = e % p, c = e / p;
-        const double *wc = W+(int64_t)perm[c]*ldw;
-        const double *ri = R2+(int64_t)i*k;
-        double s = 0.0;
-        for(int l = 0; l < p; l++)
-            s += ri[l]*wc[l];
-        double sg = sig[perm[c]];
-        Jr[e] = s/(sg*sg);
+        X[e] = W[(int64_t)perm[c]*ldw+i]/sig[perm[c]];
     }
 }
 

@@ -10,8 +10,8 @@
 //   COEF        C[j0 x jb] = Q[:, :j0]^T W, W = Q[:, j0:j0+jb]
 //   UPDATE      W -= Q[:, :j0] C
 //   R2          R2[p x p] = Q2^T Bt[:, :p]
-//   FORM_U      U[m x r] = Q[:, :p] J_r
-//   FORM_V      V[n x r] = Q2 W_r
+//   FORM_U      U[m x r] = Q[:, :p] X_r
+//   FORM_V      V[n x r] = Bt[:, :p] X_r
 #include "hip_common.h"
 #include "pipeline.h"
 
@@ -89,14 +89,13 @@ __device__ inline Op setup(const TileDesc &t, const GemmArgs &g)
         case GM_FORM_U:
             o.M = t.m; o.N = t.rank; o.K = t.p;
             o.A = ws+t.q_off; o.lda = t.m;
-            o.B = ws+t.w_off+(int64_t)t.p*(t.rank > 0 ? t.rank : 0);
-            o.ldb = t.p;
+            o.B = ws+t.w_off; o.ldb = t.p;
             o.C = g.out+t.u_off; o.ldc = t.m;
             o.valid = t.rank > 0;
             break;
         default: // GM_FORM_V
             o.M = t.n; o.N = t.rank; o.K = t.p;
-            o.A = ws+t.q2_off; o.lda = t.n;
+            o.A = ws+t.bt_off; o.lda = t.n;
             o.B = ws+t.w_off; o.ldb = t.p;
             o.C = g.out+t.v_off; o.ldc = t.n;
             o.valid = t.rank > 0;

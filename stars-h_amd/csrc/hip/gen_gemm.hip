@@ -57,6 +57,8 @@ __global__ __launch_bounds__(256) void gen_gemm_kernel(const TileDesc *tiles,
     if(bn*BN >= N)
         return;
     const int row0 = bm*BM, col0 = bn*BN;
+    // 16-column fragments that hold columns of this tile (wave-uniform)
+    const int nf_live = (N-col0+15)/16 < NF ? (N-col0+15)/16 : NF;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
     __shared__ double Bs[2][BK][BN+2];
@@ -94,12 +96,38 @@ __global__ __launch_bounds__(256) void gen_gemm_kernel(const TileDesc *tiles,
             Bs[buf][bk][bj+16*s] = breg[s];
     };
     auto gen_a = [&](int k0) {
-#pragma unroll
-        for(int q = 0; q < 4; q++)
+        if constexpr(BN <= 128)
         {
-            const int k = k0+4*q+kl;
-            areg[q] = (row_ok && k < K) ? kval<NDIM, FAM>(t.kp, xi, xk, nk,
-                    kbase+k) : 0.0;
+            // few MFMAs per entry: issue the point loads together
+            double yv[4][NDIM];
+#pragma unroll
+            for(int q = 0; q < 4; q++)
+            {
+                const int k = k0+4*q+kl;
+                const int64_t j = kbase+(k < K ? k : K-1);
+#pragma unroll
+                for(int d = 0; d < NDIM; d++)
+                    yv[q][d] = xk[d*nk+j];
+            }
+#pragma unroll
+            for(int q = 0; q < 4; q++)
+            {
+                const int k = k0+4*q+kl;
+                const double v = kval_pt<NDIM, FAM>(t.kp, xi, yv[q]);
+                areg[q] = (row_ok && k < K) ? v : 0.0;
+            }
+        }
+        else
+        {
+            // MFMA-bound: keep registers for the accumulators
+#pragma unroll
+            for(int q = 0; q < 4; q++)
+            {
+                const int k = k0+4*q+kl;
+                const double v = kval<NDIM, FAM>(t.kp, xi, xk, nk,
+                        kbase+(k < K ? k : K-1));
+                areg[q] = (row_ok && k < K) ? v : 0.0;
+            }
         }
     };
 
@@ -120,15 +148,34 @@ __global__ __launch_bounds__(256) void gen_gemm_kernel(const TileDesc *tiles,
             load_b(k0+BK);
             gen_a(k0+BK);
         }
-#pragma unroll
-        for(int q = 0; q < 4; q++)
+        if constexpr(BN <= 128)
         {
+            // fragments past the tile's own width are skipped (uniform)
 #pragma unroll
             for(int f = 0; f < NF; f++)
+                if(f < nf_live)
+                {
+#pragma unroll
+                    for(int q = 0; q < 4; q++)
+                    {
+                        const double b = Bs[buf][4*q+kl][f*16+(lane & 15)];
+                        acc[f] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                acur[q], b, acc[f], 0, 0, 0);
+                    }
+                }
+        }
+        else
+        {
+#pragma unroll
+            for(int q = 0; q < 4; q++)
             {
-                const double b = Bs[buf][4*q+kl][f*16+(lane & 15)];
-                acc[f] = __builtin_amdgcn_mfma_f64_16x16x4f64(acur[q], b,
-                        acc[f], 0, 0, 0);
+#pragma unroll
+                for(int f = 0; f < NF; f++)
+                {
+                    const double b = Bs[buf][4*q+kl][f*16+(lane & 15)];
+                    acc[f] = __builtin_amdgcn_mfma_f64_16x16x4f64(acur[q], b,
+                            acc[f], 0, 0, 0);
+                }
             }
         }
         if(more)

@@ -28,6 +28,9 @@ struct KernelParams
     double noise;
     double wave_k;
     double diag;
+    double ibexp;  // 1/bexp, 1/bsqr, 1/beta (kval)
+    double ibsqr;
+    double ibeta;
 };
 
 static inline KernelParams make_params(const starsh_hip_kernel_desc *kd)
@@ -44,6 +47,9 @@ static inline KernelParams make_params(const starsh_hip_kernel_desc *kd)
     p.noise = kd->noise;
     p.wave_k = kd->wave_k;
     p.diag = kd->diag;
+    p.ibexp = 1.0/p.bexp;
+    p.ibsqr = 1.0/p.bsqr;
+    p.ibeta = 1.0/p.beta;
     return p;
 }
 
@@ -61,12 +67,12 @@ struct TileDesc
     int64_t bt_off;    // Bt = A^T Q (n x k, ld n)
     int64_t q2_off;    // Q2 (n x k, ld n)
     int64_t r_off;     // R2 (k x k, ld k)
-    int64_t w_off;     // W_r (p x r, ld p), then J_r after it
+    int64_t w_off;     // X_r (p x r, ld p): left singular vectors of B
     int64_t s_off;     // singular values, k entries
     int64_t coef_off;  // projection coefficients (k x 32, ld k)
     int64_t u_off;     // U (m x maxrank, ld m)
     int64_t v_off;     // V (n x maxrank, ld n)
-    int64_t om8_off;   // integer test matrix (bytes, n x k, ld n)
+    int64_t om8_off;   // integer test matrix (bytes, n x k, ld n up to 64)
     int kc;           // kept (numerically independent) sketch columns
     int p;             // size of the core problem after truncation
     int rank;          // chosen rank, -1 for a false far-field tile
@@ -154,64 +160,69 @@ __device__ inline double kernel_value(const KernelParams &kp,
     return v;
 }
 
-// kernel_value with the family fixed at compile time and the row point
-// held in registers (x_i); same operations and order.
+// Entry A(i, j) for the compression GEMMs: the family fixed at compile
+// time, the row point x_i held in registers, reciprocals of the kernel
+// parameters instead of divisions and contraction into fused multiply-adds
+// allowed.  It agrees with kernel_value to a few units in the last place;
+// tiles handed back to the caller (near field, error check) use
+// kernel_value.
 template <int NDIM, int FAM>
-__device__ inline double kval(const KernelParams &kp, const double (&xi)[NDIM],
-        const double *__restrict__ yc, int64_t ny, int64_t j)
+__device__ inline double kval_pt(const KernelParams &kp,
+        const double (&xi)[NDIM], const double (&yj)[NDIM])
 {
-#pragma clang fp contract(off)
     double dist = 0.0;
 #pragma unroll
     for(int d = 0; d < NDIM; d++)
     {
-        double t = xi[d]-yc[d*ny+j];
-        dist += t*t;
+        const double t = xi[d]-yj[d];
+        dist = fma(t, t, dist);
     }
     double v;
     switch(FAM)
     {
         case STARSH_HIP_EXP:
-            dist = sqrt(dist)/kp.bexp;
-            v = dist == 0 ? kp.sigma+kp.noise : kp.sigma*exp(dist);
+            v = dist == 0 ? kp.sigma+kp.noise :
+                kp.sigma*exp(sqrt(dist)*kp.ibexp);
             break;
         case STARSH_HIP_SQREXP:
-            dist = dist/kp.bsqr;
-            v = dist == 0 ? kp.sigma+kp.noise : kp.sigma*exp(dist);
+            v = dist == 0 ? kp.sigma+kp.noise : kp.sigma*exp(dist*kp.ibsqr);
             break;
         case STARSH_HIP_MATERN:
-            dist = sqrt(dist)*kp.theta;
             v = dist == 0 ? kp.sigma+kp.noise :
-                kp.sigma*matern_closed(kp.nu, dist);
+                kp.sigma*matern_closed(kp.nu, sqrt(dist)*kp.theta);
             break;
         case STARSH_HIP_MATERN2:
-            dist = sqrt(dist)/kp.beta;
             v = dist == 0 ? kp.sigma+kp.noise :
-                kp.sigma*matern_closed(kp.nu, dist);
+                kp.sigma*matern_closed(kp.nu, sqrt(dist)*kp.ibeta);
             break;
         case STARSH_HIP_COULOMB:
-            v = dist == 0 ? 0.0 : 1.0/sqrt(dist);
+            v = dist == 0 ? 0.0 : rsqrt(dist);
             break;
         case STARSH_HIP_SIN:
-            if(dist == 0)
-                v = kp.diag;
-            else
-            {
-                dist = sqrt(dist);
-                v = sin(kp.wave_k*dist)/dist;
-            }
+        {
+            const double ri = rsqrt(dist);
+            v = dist == 0 ? kp.diag : sin(kp.wave_k*(dist*ri))*ri;
             break;
+        }
         default:
-            if(dist == 0)
-                v = kp.diag;
-            else
-            {
-                dist = sqrt(dist);
-                v = cos(kp.wave_k*dist)/dist;
-            }
+        {
+            const double ri = rsqrt(dist);
+            v = dist == 0 ? kp.diag : cos(kp.wave_k*(dist*ri))*ri;
             break;
+        }
     }
     return v;
+}
+
+template <int NDIM, int FAM>
+__device__ inline double kval(const KernelParams &kp, const double (&xi)[NDIM],
+        const double *__restrict__ yc, int64_t ny, int64_t j)
+{
+    double yj[NDIM];
+#pragma unroll
+    for(int d = 0; d < NDIM; d++)
+        yj[d] = yc[d*ny+j];
+    return kval_pt<NDIM, FAM>(kp, xi, yj);
 }
 
 // Points of a batch: row and column point sets in cluster order, SoA.

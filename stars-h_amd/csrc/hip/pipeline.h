@@ -62,7 +62,8 @@ int pipeline_truncate(TileDesc *d_tiles, int ntiles, double *ws,
         const double *d_bnorm, int kmax, double eta, int nmax,
         hipStream_t s);
 
-// Integer test matrix (n x k bytes) for the int8 sketch.
+// Integer test matrix for the int8 sketch: k columns of n bytes, leading
+// dimension n rounded up to 64, zero padded.
 int pipeline_omega_i8(signed char *d_out, int ncols, int k, hipStream_t s);
 
 // Row scales and Y = A Omega with int8 digit planes (mixed form).
@@ -71,6 +72,6 @@ int pipeline_sketch_i8(const TileDesc *d_tiles, int ntiles, int mmax,
         hipStream_t s);
 
 // One-sided Jacobi SVD of R2, singular values, rank by the reference rule
-// (each tile's own maxrank and tol), W_r and J_r.
+// (each tile's own maxrank and tol) and X_r.
 int pipeline_core(TileDesc *d_tiles, int ntiles, double *ws, int pmax,
         hipStream_t s);

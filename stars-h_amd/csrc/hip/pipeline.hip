@@ -2,7 +2,7 @@
 // tiles (src/backends/sequential/dense/drsdd.c:16-93 applied to every tile
 // of src/backends/openmp/blrm/drsdd.c:94-145):
 //   Y = A Omega; Q = orth(Y); Bt = A^T Q; truncate; Bt[:, :p] = Q2 R2;
-//   Jacobi SVD of R2; rank; U = Q J_r; V = Q2 W_r.
+//   Jacobi SVD of R2^T; rank; U = Q X_r; V = Bt X_r.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -262,7 +262,7 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
             if(in->sketch_int8)
             {
                 omegas[key] = off8;
-                off8 += ((int64_t)h[t].n*h[t].k+15)/16*16;
+                off8 += (int64_t)((h[t].n+63)/64*64)*h[t].k;
             }
             else
             {

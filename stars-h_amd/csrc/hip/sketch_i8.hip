@@ -27,15 +27,26 @@ constexpr int APAD = 80;         // bytes per LDS row (64 + 16 padding)
 
 __constant__ uint64_t c_apow_i8[64];
 
-// Integer test matrix from the dlarnv stream (see runtime.hip).
-__global__ void omega_i8_kernel(signed char *out, int64_t count)
+// Integer test matrix from the dlarnv stream (see runtime.hip), stored
+// column by column with the leading dimension ncols rounded up to 64 and
+// zeros in the padding, so that every 16-byte K slice of a column is an
+// aligned vector load.
+__global__ void omega_i8_kernel(signed char *out, int ncols, int ldo,
+        int64_t count)
 {
     const uint64_t mask = (1ULL << 48)-1;
     const uint64_t a = 33952834046453ULL;
     const double twopi = 6.2831853071795864769252867663;
-    int64_t e = (int64_t)blockIdx.x*blockDim.x+threadIdx.x;
-    if(e >= count)
+    int64_t p = (int64_t)blockIdx.x*blockDim.x+threadIdx.x;
+    if(p >= count)
         return;
+    const int64_t col = p/ldo, row = p-col*ldo;
+    if(row >= ncols)
+    {
+        out[p] = 0;
+        return;
+    }
+    const int64_t e = col*ncols+row;   // index in the reference's n x k
     uint64_t q = 2*(uint64_t)e+1, s = 1;
     for(int b = 0; q != 0; b++, q >>= 1)
         if(q & 1)
@@ -46,7 +57,7 @@ __global__ void omega_i8_kernel(signed char *out, int64_t count)
     double x = sqrt(-2.0*log(u1))*cos(twopi*u2);
     double v = rint(32.0*x);
     v = v > 127.0 ? 127.0 : (v < -127.0 ? -127.0 : v);
-    out[e] = (signed char)v;
+    out[p] = (signed char)v;
 }
 
 __device__ inline int pack4(int a, int b, int c, int d)
@@ -81,6 +92,7 @@ __global__ __launch_bounds__(256) void sketch_i8_kernel(const TileDesc *tiles,
     for(int d = 0; d < NDIM; d++)
         xi[d] = row_ok ? t.xr[d*t.nr+t.r0+row0+gi] : 0.0;
     const signed char *om = om8+t.om8_off;
+    const int ldo = (t.n+63)/64*64;
     double yacc[NF][4];
 #pragma unroll
     for(int c = 0; c < NF; c++)
@@ -90,14 +102,25 @@ __global__ __launch_bounds__(256) void sketch_i8_kernel(const TileDesc *tiles,
     for(int k0 = 0; k0 < t.n; k0 += KB)
     {
         // ---- A: FP64 entries of the chunk, scale, four int8 digit planes
+        // column points of this thread's 16 entries, loaded together
+        double yv[16][NDIM];
+#pragma unroll
+        for(int e = 0; e < 16; e++)
+        {
+            const int kk = k0+gk+e;
+            const int64_t jj = t.c0+(kk < t.n ? kk : t.n-1);
+#pragma unroll
+            for(int d = 0; d < NDIM; d++)
+                yv[e][d] = t.xc[d*t.nc+jj];
+        }
         double a[16];
         double mx = 0.0;
 #pragma unroll
         for(int e = 0; e < 16; e++)
         {
             const int kk = k0+gk+e;
-            a[e] = (row_ok && kk < t.n) ? kval<NDIM, FAM>(t.kp, xi, t.xc,
-                    t.nc, t.c0+kk) : 0.0;
+            const double v = kval_pt<NDIM, FAM>(t.kp, xi, yv[e]);
+            a[e] = (row_ok && kk < t.n) ? v : 0.0;
             const double f = fabs(a[e]);
             mx = f > mx ? f : mx;
         }
@@ -136,30 +159,15 @@ __global__ __launch_bounds__(256) void sketch_i8_kernel(const TileDesc *tiles,
             v.w = pack4(dg[s][12], dg[s][13], dg[s][14], dg[s][15]);
             *(int4v *)&As[s][gi*APAD+gk] = v;
         }
-        // ---- B: Omega columns col0.., K rows k0.. (column-major, ld n)
+        // ---- B: Omega columns col0.., K rows k0.. (column-major, leading
+        // dimension ldo = n rounded up to 64, zero padded)
 #pragma unroll
         for(int h = 0; h < CB/64; h++)
         {
             const int j = (tid >> 2)+64*h, kq = (tid & 3)*16;
             int4v v = int4v{0, 0, 0, 0};
             if(col0+j < t.k)
-            {
-                const signed char *src = om+(int64_t)(col0+j)*t.n+k0+kq;
-                int w[4];
-#pragma unroll
-                for(int q = 0; q < 4; q++)
-                {
-                    int b[4];
-#pragma unroll
-                    for(int z = 0; z < 4; z++)
-                    {
-                        const int kk = k0+kq+4*q+z;
-                        b[z] = kk < t.n ? src[4*q+z] : 0;
-                    }
-                    w[q] = pack4(b[0], b[1], b[2], b[3]);
-                }
-                v = int4v{w[0], w[1], w[2], w[3]};
-            }
+                v = *(const int4v *)(om+(int64_t)(col0+j)*ldo+k0+kq);
             *(int4v *)&Bs[j*APAD+kq] = v;
         }
         __syncthreads();
@@ -229,11 +237,12 @@ int pipeline_omega_i8(signed char *d_out, int ncols, int k, hipStream_t s)
         HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_apow_i8), tb, sizeof(tb)));
         ready = true;
     }
-    int64_t count = (int64_t)ncols*k;
+    const int ldo = (ncols+63)/64*64;
+    int64_t count = (int64_t)ldo*k;
     if(count <= 0)
         return STARSH_SUCCESS;
     hipLaunchKernelGGL(omega_i8_kernel, dim3((unsigned)((count+255)/256)),
-            dim3(256), 0, s, d_out, count);
+            dim3(256), 0, s, d_out, ncols, ldo, count);
     HIP_TRY(hipGetLastError());
     return STARSH_SUCCESS;
 }
