@@ -402,7 +402,7 @@ static size_t tile_bytes(int m, int n, int k, int maxrank)
 {
     size_t r = (size_t)(maxrank < k ? maxrank : k);
     return sizeof(double)*((size_t)m*k+2*(size_t)n*k+4*(size_t)k*k+
-            17*(size_t)k+(size_t)(m+n)*r+(size_t)m)+(size_t)n*k;
+            (size_t)k+(size_t)(m+n)*r)+(size_t)(n+63)*k;
 }
 
 int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,

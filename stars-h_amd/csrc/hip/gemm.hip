@@ -7,8 +7,6 @@
 //   SKETCH_MEM  Y[m x k]  = A[m x n] Omega[n x k], A explicit in memory
 //   PROJ_MEM    Bt[n x kc] = A^T[n x m] Q[m x kc], A explicit
 // (tiles generated from points use gen_gemm.hip instead)
-//   COEF        C[j0 x jb] = Q[:, :j0]^T W, W = Q[:, j0:j0+jb]
-//   UPDATE      W -= Q[:, :j0] C
 //   R2          R2[p x p] = Q2^T Bt[:, :p]
 //   FORM_U      U[m x r] = Q[:, :p] X_r
 //   FORM_V      V[n x r] = Bt[:, :p] X_r
@@ -40,10 +38,6 @@ __device__ inline Op setup(const TileDesc &t, const GemmArgs &g)
     o.alpha = 1.0;
     o.beta = 0.0;
     double *ws = g.ws;
-    const bool q2 = g.which != 0;
-    const int rows = q2 ? t.n : t.m;
-    const int64_t qoff = q2 ? t.q2_off : t.q_off;
-    const int width = q2 ? t.p : t.k;
     switch(MODE)
     {
         case GM_SKETCH_MEM:
@@ -58,27 +52,6 @@ __device__ inline Op setup(const TileDesc &t, const GemmArgs &g)
             o.B = ws+t.q_off; o.ldb = t.m;
             o.C = ws+t.bt_off; o.ldc = t.n;
             break;
-        case GM_COEF:
-        {
-            int jb = width-g.j0 < g.jb ? width-g.j0 : g.jb;
-            o.M = g.j0; o.N = jb; o.K = rows;
-            o.A = ws+qoff; o.lda = rows;
-            o.B = ws+qoff+(int64_t)g.j0*rows; o.ldb = rows;
-            o.C = ws+t.coef_off; o.ldc = t.k;
-            o.valid = jb > 0 && g.j0 > 0 && !(t.flags & (q2 ? 2 : 1));
-            break;
-        }
-        case GM_UPDATE:
-        {
-            int jb = width-g.j0 < g.jb ? width-g.j0 : g.jb;
-            o.M = rows; o.N = jb; o.K = g.j0;
-            o.A = ws+qoff; o.lda = rows;
-            o.B = ws+t.coef_off; o.ldb = t.k;
-            o.C = ws+qoff+(int64_t)g.j0*rows; o.ldc = rows;
-            o.alpha = -1.0; o.beta = 1.0;
-            o.valid = jb > 0 && g.j0 > 0 && !(t.flags & (q2 ? 2 : 1));
-            break;
-        }
         case GM_R2:
             o.M = t.p; o.N = t.p; o.K = t.n;
             o.A = ws+t.q2_off; o.lda = t.n;
@@ -109,8 +82,6 @@ __device__ inline Op setup(const TileDesc &t, const GemmArgs &g)
 template <int MODE> struct ModeTraits;
 template <> struct ModeTraits<GM_SKETCH_MEM> { static constexpr int a = 0; };
 template <> struct ModeTraits<GM_PROJ_MEM> { static constexpr int a = 1; };
-template <> struct ModeTraits<GM_COEF> { static constexpr int a = 1; };
-template <> struct ModeTraits<GM_UPDATE> { static constexpr int a = 0; };
 template <> struct ModeTraits<GM_R2> { static constexpr int a = 1; };
 template <> struct ModeTraits<GM_FORM_U> { static constexpr int a = 0; };
 template <> struct ModeTraits<GM_FORM_V> { static constexpr int a = 0; };
@@ -251,10 +222,6 @@ int pipeline_gemm(int mode, const TileDesc *d_tiles, int ntiles, int mmax,
                                     mmax, nmax, g, s);
         case GM_PROJ_MEM: return launch<GM_PROJ_MEM>(d_tiles, ntiles, mmax,
                                   nmax, g, s);
-        case GM_COEF: return launch<GM_COEF>(d_tiles, ntiles, mmax, nmax, g,
-                              s);
-        case GM_UPDATE: return launch<GM_UPDATE>(d_tiles, ntiles, mmax, nmax,
-                                g, s);
         case GM_R2: return launch<GM_R2>(d_tiles, ntiles, mmax, nmax, g, s);
         case GM_FORM_U: return launch<GM_FORM_U>(d_tiles, ntiles, mmax, nmax,
                                 g, s);

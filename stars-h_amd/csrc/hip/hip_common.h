@@ -69,7 +69,6 @@ struct TileDesc
     int64_t r_off;     // R2 (k x k, ld k)
     int64_t w_off;     // X_r (p x r, ld p): left singular vectors of B
     int64_t s_off;     // singular values, k entries
-    int64_t coef_off;  // projection coefficients (k x 32, ld k)
     int64_t u_off;     // U (m x maxrank, ld m)
     int64_t v_off;     // V (n x maxrank, ld n)
     int64_t om8_off;   // integer test matrix (bytes, n x k, ld n up to 64)

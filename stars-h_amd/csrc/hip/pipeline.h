@@ -7,8 +7,6 @@ enum GemmMode
 {
     GM_SKETCH_MEM = 0,
     GM_PROJ_MEM,
-    GM_COEF,
-    GM_UPDATE,
     GM_R2,
     GM_FORM_U,
     GM_FORM_V
@@ -20,8 +18,6 @@ struct GemmArgs
     double *out;          // output base for U / V
     const double *abase;  // explicit tiles base (dense mode)
     int ndim, family;     // kernel of every tile of the batch
-    int j0, jb;           // panel position (COEF / UPDATE)
-    int which;            // 0: Q (m rows), 1: Q2 (n rows)
 };
 
 int pipeline_gemm(int mode, const TileDesc *d_tiles, int ntiles, int mmax,
@@ -37,8 +33,9 @@ int pipeline_gen_gemm(int proj, const TileDesc *d_tiles, int ntiles,
 int pipeline_colnorm(const TileDesc *d_tiles, int ntiles, int which,
         double *ws, double *d_norm, int kmax, hipStream_t s);
 
-// One block classical Gram-Schmidt panel pass with intra-panel CGS2 and
-// deflation.  which: 0 = Q (width k), 1 = Q2 (width p).  Columns whose
+// One block Gram-Schmidt pass for a panel of jb columns at j0 (projection
+// against the previous columns, then CGS2 inside the panel with
+// deflation).  which: 0 = Q (width k), 1 = Q2 (width p).  Columns whose
 // norm after projection is at most tau * ref (ref = d_ref[t*kmax+j], or 1
 // when d_ref is null) are set to zero and flagged in d_keep.  The first
 // pass (d_ref not null) marks a tile whose whole panel was dropped as done;

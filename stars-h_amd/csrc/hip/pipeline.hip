@@ -114,12 +114,9 @@ ActiveProbe probe;
 
 int bcgs2(TileDesc *d_tiles, int ntiles, int which, double *ws,
         const double *d_ref, int kmax, int width_max, int rows_max,
-        int *d_keep, double tau_first, int *d_active, GemmArgs g,
-        hipStream_t s)
+        int *d_keep, double tau_first, int *d_active, hipStream_t s)
 {
     const int pb = rows_max <= 2048 ? 8 : 4;
-    g.which = which;
-    g.jb = pb;
     TRY(probe.init());
     TRY(pipeline_count_active(d_tiles, ntiles, which, d_active, s));
     int panel = 0;
@@ -132,20 +129,11 @@ int bcgs2(TileDesc *d_tiles, int ntiles, int which, double *ws,
             if(probe.h[slot] == 0)
                 break;
         }
-        g.j0 = j0;
         for(int pass = 0; pass < 2; pass++)
-        {
-            if(j0 > 0)
-            {
-                TRY(pipeline_gemm(GM_COEF, d_tiles, ntiles, j0, pb, g, s));
-                TRY(pipeline_gemm(GM_UPDATE, d_tiles, ntiles, rows_max, pb, g,
-                            s));
-            }
             TRY(pipeline_panel(d_tiles, ntiles, which, ws, j0, pb,
                         pass == 0 ? d_ref : nullptr, kmax,
                         pass == 0 ? tau_first : TAU_SECOND, d_keep, rows_max,
                         d_active, s));
-        }
         const int slot = panel % NSLOT;
         HIP_TRY(hipMemcpyAsync(probe.h+slot, d_active, sizeof(int),
                     hipMemcpyDeviceToHost, s));
@@ -287,7 +275,6 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
         d.r_off = off; off += 2*k*k;
         d.w_off = off; off += 2*k*k;
         d.s_off = off; off += k;
-        d.coef_off = off; off += k*16;
         d.u_off = uvoff; uvoff += m*rmax;
         d.v_off = uvoff; uvoff += n*rmax;
         out->uoff[t] = d.u_off;
@@ -356,7 +343,7 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     HIP_TRY(hipMemsetAsync(keep.p, 0, sizeof(int)*(size_t)nt*kmax, s));
     TRY(bcgs2(d_tiles, nt, 0, d_ws, (double *)norms.p, kmax, kmax, mmax,
                 (int *)keep.p, in->tau > 0 ? in->tau : TAU_FIRST,
-                (int *)active.p, g, s));
+                (int *)active.p, s));
     dump("Qfull", d_ws+h[0].q_off, (int64_t)h[0].m*h[0].k, s);
     TRY(pipeline_compact(d_tiles, nt, d_ws, (int *)keep.p, kmax, s));
     dump("Q", d_ws+h[0].q_off, (int64_t)h[0].m*h[0].k, s);
@@ -380,7 +367,7 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     if(pmax > 0)
     {
         TRY(bcgs2(d_tiles, nt, 1, d_ws, (double *)norms.p, kmax, pmax, nmax,
-                    (int *)keep.p, TAU_FIRST, (int *)active.p, g, s));
+                    (int *)keep.p, TAU_FIRST, (int *)active.p, s));
         dump("Q2", d_ws+h[0].q2_off, (int64_t)h[0].n*h[0].p, s);
         TRY(pipeline_gemm(GM_R2, d_tiles, nt, pmax, pmax, g, s));
         dump("R2", d_ws+h[0].r_off, (int64_t)h[0].k*h[0].k, s);

@@ -1,6 +1,6 @@
-// Orthogonalisation stages of the randomized SVD: block classical
-// Gram-Schmidt applied twice (BCGS2) with column deflation, compaction of the
-// kept columns, and truncation of the projected matrix.
+// Orthogonalisation stages of the randomized SVD: block Gram-Schmidt
+// applied twice (BCGS2) with column deflation, compaction of the kept
+// columns, and truncation of the projected matrix.
 //
 // The reference factors the sketch with Householder QR (dgeqrf + dorgqr,
 // src/backends/sequential/dense/drsdd.c:64-67).  The sketch of a smooth
@@ -17,8 +17,6 @@
 
 namespace {
 
-constexpr int PANEL_THREADS = 256;
-constexpr int NWAVES = PANEL_THREADS/64;
 
 __device__ inline double wave_sum(double v)
 {
@@ -49,19 +47,83 @@ __global__ __launch_bounds__(256) void colnorm_kernel(const TileDesc *tiles,
     }
 }
 
-// Intra-panel CGS2 with deflation.  The panel (rows x PB) lives in
-// registers: thread r holds rows r, r+256, ... (RPT of them).  Each dot
-// product is reduced within the wave by shuffles; the four wave partials go
-// to LDS and every thread sums them itself after one barrier.  The partial
-// buffers alternate between consecutive reductions, so a buffer is only
-// rewritten after the barrier that follows its last read.  Blocks are small
-// so that several tiles share a CU and hide each other's reductions.
-template <int RPT, int PB>
-__global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(
-        TileDesc *tiles, int which, double *ws, int j0, const double *ref,
-        int kmax, double tau2, int *keep, int first_pass, int *active)
+// Sum over the 64 lanes of a wave of each of V values (V a power of two,
+// at most 64) by recursive halving: at every step a lane keeps half of its
+// values and adds its partner's copy of them, so V-1+log2(64/V) shuffles
+// reduce all V.  On return the lane holds the total of value index
+// halving_index<V>(lane).  The steps are unrolled by template recursion.
+template <int H, int O>
+struct Halve
 {
-    __shared__ double red[2][NWAVES][PB];
+    template <int V>
+    static __device__ inline void run(double (&v)[V], int lane)
+    {
+        const bool up = (lane & O) != 0;
+#pragma unroll
+        for(int i = 0; i < H; i++)
+        {
+            const double send = up ? v[i] : v[i+H];
+            const double keep = up ? v[i+H] : v[i];
+            v[i] = keep+__shfl_xor(send, O, 64);
+        }
+        Halve<H/2, O/2>::run(v, lane);
+    }
+};
+
+template <int O>
+struct Halve<0, O>
+{
+    template <int V>
+    static __device__ inline void run(double (&v)[V], int lane)
+    {
+        (void)lane;
+#pragma unroll
+        for(int o = O; o >= 1; o >>= 1)
+            v[0] += __shfl_xor(v[0], o, 64);
+    }
+};
+
+template <int V>
+__device__ inline double halving_sum(double (&v)[V], int lane)
+{
+    Halve<V/2, 32>::run(v, lane);
+    return v[0];
+}
+
+template <int V>
+__device__ inline int halving_index(int lane)
+{
+    int idx = 0;
+#pragma unroll
+    for(int st = 0; (V >> (st+1)) >= 1; st++)
+        if(lane & (32 >> st))
+            idx += V >> (st+1);
+    return idx;
+}
+
+// One pass of block Gram-Schmidt for the panel W = columns j0..j0+PB-1 of
+// Q (which = 0, m rows, width k) or Q2 (which = 1, n rows, width p), fused
+// in one workgroup per tile:
+//   1. projection against the previous columns, CH at a time (block
+//      modified Gram-Schmidt): C = Q_chᵀ W, W -= Q_ch C; each chunk of Q is
+//      read once, and its CH x PB dot products are reduced by
+//      halving_sum, then across the waves in LDS;
+//   2. CGS2 inside the panel with deflation: a column whose norm after
+//      projection is at most tau times its original norm (ref, or 1 when
+//      ref is null) is set to zero and flagged in keep.
+// The panel lives in registers: thread t holds rows t, t+NT, ... (RPT of
+// them).  Called twice per panel (BCGS2); the first call (ref != null)
+// marks a tile whose whole panel was dropped as done, and *active counts
+// the tiles that still have panels to process.
+template <int NT, int RPT, int PB>
+__global__ __launch_bounds__(NT) void bcgs_kernel(TileDesc *tiles, int which,
+        double *ws, int j0, const double *ref, int kmax, double tau2,
+        int *keep, int first_pass, int *active)
+{
+    constexpr int NW = NT/64, CH = 4, V = CH*PB;
+    __shared__ double red[NW][V];
+    __shared__ double fin[V];
+    __shared__ double pred[2][NW][PB];
     TileDesc &td = tiles[blockIdx.x];
     const bool q2 = which != 0;
     const int done_bit = q2 ? 2 : 1;
@@ -79,16 +141,68 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(
         return;
     }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double *W = ws+(q2 ? td.q2_off : td.q_off)+(int64_t)j0*rows;
+    double *Qb = ws+(q2 ? td.q2_off : td.q_off);
+    double *W = Qb+(int64_t)j0*rows;
     double w[RPT][PB];
 #pragma unroll
     for(int r = 0; r < RPT; r++)
     {
-        const int i = threadIdx.x+r*PANEL_THREADS;
+        const int i = threadIdx.x+r*NT;
 #pragma unroll
         for(int c = 0; c < PB; c++)
             w[r][c] = (i < rows && c < jb) ? W[(int64_t)c*rows+i] : 0.0;
     }
+    // ---- 1. projection against columns 0..j0-1 (j0 is a multiple of CH)
+    const int my_idx = halving_index<V>(lane);
+    const bool writer = (lane & ((64/V)-1)) == 0;
+    for(int l0 = 0; l0 < j0; l0 += CH)
+    {
+        double q[RPT][CH];
+#pragma unroll
+        for(int r = 0; r < RPT; r++)
+        {
+            const int i = threadIdx.x+r*NT;
+#pragma unroll
+            for(int c = 0; c < CH; c++)
+                q[r][c] = i < rows ? Qb[(int64_t)(l0+c)*rows+i] : 0.0;
+        }
+        double v[V];
+#pragma unroll
+        for(int c = 0; c < CH; c++)
+#pragma unroll
+            for(int b = 0; b < PB; b++)
+            {
+                double d = 0.0;
+#pragma unroll
+                for(int r = 0; r < RPT; r++)
+                    d += q[r][c]*w[r][b];
+                v[c*PB+b] = d;
+            }
+        const double tot = halving_sum<V>(v, lane);
+        if(writer)
+            red[wave][my_idx] = tot;
+        __syncthreads();
+        if(threadIdx.x < V)
+        {
+            double sum = 0.0;
+#pragma unroll
+            for(int q2w = 0; q2w < NW; q2w++)
+                sum += red[q2w][threadIdx.x];
+            fin[threadIdx.x] = sum;
+        }
+        __syncthreads();
+#pragma unroll
+        for(int c = 0; c < CH; c++)
+#pragma unroll
+            for(int b = 0; b < PB; b++)
+            {
+                const double coef = fin[c*PB+b];
+#pragma unroll
+                for(int r = 0; r < RPT; r++)
+                    w[r][b] -= q[r][c]*coef;
+            }
+    }
+    // ---- 2. CGS2 inside the panel with deflation
     int ndropped = 0;
     int rb = 0;   // partial buffer of the next reduction
 #pragma unroll
@@ -111,7 +225,7 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(
                                 d += w[r][l]*w[r][c];
                             d = wave_sum(d);
                             if(lane == 0)
-                                red[rb][wave][l] = d;
+                                pred[rb][wave][l] = d;
                         }
                     __syncthreads();
 #pragma unroll
@@ -120,8 +234,8 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(
                         {
                             double coef = 0.0;
 #pragma unroll
-                            for(int q = 0; q < NWAVES; q++)
-                                coef += red[rb][q][l];
+                            for(int q2w = 0; q2w < NW; q2w++)
+                                coef += pred[rb][q2w][l];
 #pragma unroll
                             for(int r = 0; r < RPT; r++)
                                 w[r][c] -= coef*w[r][l];
@@ -135,12 +249,12 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(
                 d += w[r][c]*w[r][c];
             d = wave_sum(d);
             if(lane == 0)
-                red[rb][wave][0] = d;
+                pred[rb][wave][0] = d;
             __syncthreads();
             double n2 = 0.0;
 #pragma unroll
-            for(int q = 0; q < NWAVES; q++)
-                n2 += red[rb][q][0];
+            for(int q2w = 0; q2w < NW; q2w++)
+                n2 += pred[rb][q2w][0];
             rb ^= 1;
             const double r2 = ref ? ref[(int64_t)blockIdx.x*kmax+j0+c] : 1.0;
             const bool drop = !(n2 > tau2*r2) || n2 == 0.0;
@@ -162,7 +276,7 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(
 #pragma unroll
     for(int r = 0; r < RPT; r++)
     {
-        const int i = threadIdx.x+r*PANEL_THREADS;
+        const int i = threadIdx.x+r*NT;
         if(i < rows)
 #pragma unroll
             for(int c = 0; c < PB; c++)
@@ -175,7 +289,7 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(
         // that will be skipped are zeroed here, so Q2^T Bt ignores them.
         double *rest = W+(int64_t)jb*rows;
         const int64_t cnt = (int64_t)(width-j0-jb)*rows;
-        for(int64_t e = threadIdx.x; e < cnt; e += PANEL_THREADS)
+        for(int64_t e = threadIdx.x; e < cnt; e += NT)
             rest[e] = 0.0;
     }
     if(threadIdx.x == 0)
@@ -314,17 +428,17 @@ int pipeline_panel(const TileDesc *d_tiles, int ntiles, int which,
     const int first = d_ref != nullptr;
     const double tau2 = tau*tau;
     if(rows_max <= 1024 && jb == 8)
-        hipLaunchKernelGGL((panel_kernel<4, 8>), dim3(ntiles),
-                dim3(PANEL_THREADS), 0, s, tiles, which, ws, j0, d_ref, kmax,
-                tau2, d_keep, first, d_active);
+        hipLaunchKernelGGL((bcgs_kernel<256, 4, 8>), dim3(ntiles), dim3(256),
+                0, s, tiles, which, ws, j0, d_ref, kmax, tau2, d_keep, first,
+                d_active);
     else if(rows_max <= 2048 && jb == 8)
-        hipLaunchKernelGGL((panel_kernel<8, 8>), dim3(ntiles),
-                dim3(PANEL_THREADS), 0, s, tiles, which, ws, j0, d_ref, kmax,
-                tau2, d_keep, first, d_active);
+        hipLaunchKernelGGL((bcgs_kernel<512, 4, 8>), dim3(ntiles), dim3(512),
+                0, s, tiles, which, ws, j0, d_ref, kmax, tau2, d_keep, first,
+                d_active);
     else if(rows_max <= 4096 && jb == 4)
-        hipLaunchKernelGGL((panel_kernel<16, 4>), dim3(ntiles),
-                dim3(PANEL_THREADS), 0, s, tiles, which, ws, j0, d_ref, kmax,
-                tau2, d_keep, first, d_active);
+        hipLaunchKernelGGL((bcgs_kernel<512, 8, 4>), dim3(ntiles), dim3(512),
+                0, s, tiles, which, ws, j0, d_ref, kmax, tau2, d_keep, first,
+                d_active);
     else
         return STARSH_WRONG_PARAMETER;
     HIP_TRY(hipGetLastError());
