@@ -102,13 +102,33 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--first-bucket", type=int, default=None,
                     help="force the mixed form's first rank bucket (index)")
+    ap.add_argument("--set", action="append", default=[],
+                    metavar="NAME=VALUE",
+                    help="starsh_hip_set_option before the run")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB "
+                         "(reference kernel for HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the compression path runs on a "
               "single GPU in this build (multi-GPU sharding is not built); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration > 0:
+        import torch
+        n = args.copy_calibration << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n}))
+        return
     import starsh_hip as S
+    for kv in args.set:
+        name, val = kv.split("=")
+        assert S.lib().starsh_hip_set_option(
+            name.encode(), ctypes.c_double(float(val))) == 0, kv
     if args.first_bucket is not None:
         assert S.lib().starsh_hip_set_option(
             b"mixed_first_bucket", ctypes.c_double(args.first_bucket)) == 0

@@ -16,7 +16,8 @@
 
 namespace {
 
-constexpr int CORE_THREADS = 256;
+constexpr int CORE_THREADS = 256;     // tiles whose W fits in LDS
+constexpr int CORE_THREADS_BIG = 1024; // W in global memory
 constexpr int LDS_PMAX = 128;
 
 __device__ inline double group_sum(double v, int g)
@@ -31,6 +32,7 @@ __device__ inline double group_sum(double v, int g)
 // LDS the groups are as small as needed to give every pair of a round its
 // own group; in global memory a whole wave takes a pair, so that each
 // access of a column is one coalesced 512-byte segment.
+template <int NT>
 __device__ void jacobi(double *W, int ldw, int p, bool in_lds,
         int *sweeps_out)
 {
@@ -39,9 +41,9 @@ __device__ void jacobi(double *W, int ldw, int p, bool in_lds,
     const int npairs = pe/2;
     int g = 64;
     if(in_lds)
-        while(g > 1 && g*npairs > CORE_THREADS)
+        while(g > 1 && g*npairs > NT)
             g >>= 1;
-    const int ngroups = CORE_THREADS/g;
+    const int ngroups = NT/g;
     const int grp = threadIdx.x/g, gl = threadIdx.x % g;
     const double ctol = (double)p*DBL_EPSILON;
     int sweep;
@@ -108,10 +110,12 @@ __device__ void jacobi(double *W, int ldw, int p, bool in_lds,
 }
 
 // Dynamic LDS: sig[pmax] doubles, then (when lds_p > 0) W[lds_p^2]
-// doubles, then perm[pmax] ints.  Tiles with p <= lds_p work in LDS, larger
-// ones in their workspace.
-__global__ __launch_bounds__(CORE_THREADS) void core_kernel(TileDesc *tiles,
-        double *ws, int pmax, int lds_p)
+// doubles, then perm[pmax] ints.  With big == 0 the kernel takes the tiles
+// with p <= lds_p (W in LDS); with big != 0 those with p > LDS_PMAX (W in
+// their workspace, more threads to keep many column pairs in flight).
+template <int NT>
+__global__ __launch_bounds__(NT) void core_kernel(TileDesc *tiles,
+        double *ws, int pmax, int lds_p, int big)
 {
     extern __shared__ double dyn[];
     double *sig = dyn;
@@ -121,6 +125,8 @@ __global__ __launch_bounds__(CORE_THREADS) void core_kernel(TileDesc *tiles,
     TileDesc &td = tiles[blockIdx.x];
     const int p = td.p, k = td.k, maxrank = td.maxrank;
     const double tol = td.tol;
+    if(big ? p <= LDS_PMAX : p > lds_p)
+        return;
     if(p <= 0)
     {
         // zero sketch: all singular values vanish and the rank is 0
@@ -151,7 +157,7 @@ __global__ __launch_bounds__(CORE_THREADS) void core_kernel(TileDesc *tiles,
     }
     __syncthreads();
     int sweeps;
-    jacobi(W, ldw, p, p <= lds_p, &sweeps);
+    jacobi<NT>(W, ldw, p, p <= lds_p, &sweeps);
     __syncthreads();
     // Singular values: column norms of W.
     for(int j = threadIdx.x; j < p; j += blockDim.x)
@@ -242,14 +248,21 @@ int pipeline_core(TileDesc *d_tiles, int ntiles, double *ws, int pmax,
     static bool attr = false;
     if(!attr)
     {
-        HIP_TRY(hipFuncSetAttribute((const void *)core_kernel,
+        HIP_TRY(hipFuncSetAttribute((const void *)core_kernel<CORE_THREADS>,
                     hipFuncAttributeMaxDynamicSharedMemorySize,
                     (int)(sizeof(double)*(1024+LDS_PMAX*LDS_PMAX)+
                         sizeof(int)*1024)));
         attr = true;
     }
-    hipLaunchKernelGGL(core_kernel, dim3(ntiles), dim3(CORE_THREADS), bytes,
-            s, d_tiles, ws, pm, lds_p);
+    hipLaunchKernelGGL(core_kernel<CORE_THREADS>, dim3(ntiles),
+            dim3(CORE_THREADS), bytes, s, d_tiles, ws, pm, lds_p, 0);
     HIP_TRY(hipGetLastError());
+    if(pm > LDS_PMAX)
+    {
+        const size_t bb = sizeof(double)*(size_t)pm+sizeof(int)*(size_t)pm;
+        hipLaunchKernelGGL(core_kernel<CORE_THREADS_BIG>, dim3(ntiles),
+                dim3(CORE_THREADS_BIG), bb, s, d_tiles, ws, pm, 0, 1);
+        HIP_TRY(hipGetLastError());
+    }
     return STARSH_SUCCESS;
 }

@@ -18,14 +18,62 @@ constexpr int BK = 16;
 // Column block width (0: by problem size); set by starsh_hip_set_option.
 int gen_gemm_bn = 0;
 
+// Workgroups are dispatched to the 8 XCDs of the chip round robin (group
+// L runs on XCD L % 8), and each XCD has its own L2.  The 1-D grid is
+// decoded so that an XCD sees few distinct right operands:
+//  - shared B (the sketch: every tile multiplies the same Omega): XCD x
+//    takes only column block x % nbn (when nbn divides 8), so it keeps one
+//    slice of Omega in its L2;
+//  - per-tile B (the projection: B = the tile's own Q): all blocks of a
+//    tile run on one XCD, which reads that Q once.
+constexpr int NXCD = 8;
+
+__device__ inline bool map_block(int64_t L, int ntiles, int nbm, int nbn,
+        bool shared_b, int &tile, int &bm, int &bn)
+{
+    const int xcd = (int)(L % NXCD);
+    const int64_t slot = L / NXCD;
+    if(shared_b && NXCD % nbn == 0)
+    {
+        const int per = NXCD/nbn;
+        bn = xcd % nbn;
+        const int64_t f = slot*per+xcd/nbn;
+        tile = (int)(f/nbm);
+        bm = (int)(f % nbm);
+    }
+    else
+    {
+        const int64_t X = (int64_t)nbm*nbn;
+        tile = (int)((slot/X)*NXCD+xcd);
+        const int b = (int)(slot % X);
+        bm = b % nbm;
+        bn = b / nbm;
+    }
+    return tile < ntiles;
+}
+
+inline int64_t mapped_grid(int ntiles, int nbm, int nbn, bool shared_b)
+{
+    if(shared_b && NXCD % nbn == 0)
+    {
+        const int per = NXCD/nbn;
+        return (int64_t)NXCD*(((int64_t)ntiles*nbm+per-1)/per);
+    }
+    return (int64_t)NXCD*((ntiles+NXCD-1)/NXCD)*nbm*nbn;
+}
+
 // PROJ=false: rows are row points, K runs over column points.
 // PROJ=true : rows are column points, K runs over row points (A^T).
+// gnbm x gnbn is the block grid of the largest tile of the launch.
 template <int NDIM, int FAM, int BN, bool PROJ>
 __global__ __launch_bounds__(256) void gen_gemm_kernel(const TileDesc *tiles,
-        GemmArgs g)
+        GemmArgs g, int ntiles, int gnbm, int gnbn)
 {
     constexpr int NF = BN/16;
-    const TileDesc t = tiles[blockIdx.y];
+    int tile, bm, bn;
+    if(!map_block(blockIdx.x, ntiles, gnbm, gnbn, !PROJ, tile, bm, bn))
+        return;
+    const TileDesc t = tiles[tile];
     int M, N, K, ldb, ldc;
     const double *B;
     double *C;
@@ -52,9 +100,7 @@ __global__ __launch_bounds__(256) void gen_gemm_kernel(const TileDesc *tiles,
     }
     if(M <= 0 || N <= 0)
         return;
-    const int nbm = (M+BM-1)/BM;
-    const int bm = blockIdx.x % nbm, bn = blockIdx.x / nbm;
-    if(bn*BN >= N)
+    if(bm*BM >= M || bn*BN >= N)
         return;
     const int row0 = bm*BM, col0 = bn*BN;
     // 16-column fragments that hold columns of this tile (wave-uniform)
@@ -207,24 +253,22 @@ int launch_nd(const TileDesc *d_tiles, int ntiles, int mmax, int nmax,
     const int nbm = (mmax+BM-1)/BM;
     int bn = gen_gemm_bn > 0 ? gen_gemm_bn : (nmax > 128 ? 256 :
             (nmax > 64 ? 128 : 64));
+    const int nbn = (nmax+bn-1)/bn;
+    const int64_t grid = mapped_grid(ntiles, nbm, nbn, !PROJ);
+    if(grid > 0x7fffffff)
+        return STARSH_WRONG_PARAMETER;
     if(bn == 256)
-    {
-        dim3 grid(nbm*((nmax+255)/256), ntiles);
-        hipLaunchKernelGGL((gen_gemm_kernel<NDIM, FAM, 256, PROJ>), grid,
-                dim3(256), 0, s, d_tiles, g);
-    }
+        hipLaunchKernelGGL((gen_gemm_kernel<NDIM, FAM, 256, PROJ>),
+                dim3((unsigned)grid), dim3(256), 0, s, d_tiles, g, ntiles,
+                nbm, nbn);
     else if(bn == 128)
-    {
-        dim3 grid(nbm*((nmax+127)/128), ntiles);
-        hipLaunchKernelGGL((gen_gemm_kernel<NDIM, FAM, 128, PROJ>), grid,
-                dim3(256), 0, s, d_tiles, g);
-    }
+        hipLaunchKernelGGL((gen_gemm_kernel<NDIM, FAM, 128, PROJ>),
+                dim3((unsigned)grid), dim3(256), 0, s, d_tiles, g, ntiles,
+                nbm, nbn);
     else
-    {
-        dim3 grid(nbm*((nmax+63)/64), ntiles);
-        hipLaunchKernelGGL((gen_gemm_kernel<NDIM, FAM, 64, PROJ>), grid,
-                dim3(256), 0, s, d_tiles, g);
-    }
+        hipLaunchKernelGGL((gen_gemm_kernel<NDIM, FAM, 64, PROJ>),
+                dim3((unsigned)grid), dim3(256), 0, s, d_tiles, g, ntiles,
+                nbm, nbn);
     HIP_TRY(hipGetLastError());
     return STARSH_SUCCESS;
 }

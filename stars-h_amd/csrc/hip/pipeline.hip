@@ -112,11 +112,16 @@ struct ActiveProbe
 
 ActiveProbe probe;
 
+// Panel width of the block Gram-Schmidt (4, or 8 for tiles of at most
+// 2048 rows); set by starsh_hip_set_option("qr_panel").  Narrow panels
+// measured faster on MI355X: smaller blocks, more of them per CU.
+int qr_panel_width = 4;
+
 int bcgs2(TileDesc *d_tiles, int ntiles, int which, double *ws,
         const double *d_ref, int kmax, int width_max, int rows_max,
         int *d_keep, double tau_first, int *d_active, hipStream_t s)
 {
-    const int pb = rows_max <= 2048 ? 8 : 4;
+    const int pb = rows_max <= 2048 && qr_panel_width != 4 ? 8 : 4;
     TRY(probe.init());
     TRY(pipeline_count_active(d_tiles, ntiles, which, d_active, s));
     int panel = 0;
@@ -503,4 +508,12 @@ void rsdd_release_pools(void)
     uv_pool.release();
     pack_pool.release();
     stage_pool.release();
+}
+
+extern "C" int starsh_hip_qr_panel(int width)
+{
+    if(width != 4 && width != 8)
+        return STARSH_WRONG_PARAMETER;
+    qr_panel_width = width;
+    return STARSH_SUCCESS;
 }

@@ -427,18 +427,21 @@ int pipeline_panel(const TileDesc *d_tiles, int ntiles, int which,
     TileDesc *tiles = const_cast<TileDesc *>(d_tiles);
     const int first = d_ref != nullptr;
     const double tau2 = tau*tau;
-    if(rows_max <= 1024 && jb == 8)
-        hipLaunchKernelGGL((bcgs_kernel<256, 4, 8>), dim3(ntiles), dim3(256),
-                0, s, tiles, which, ws, j0, d_ref, kmax, tau2, d_keep, first,
-                d_active);
-    else if(rows_max <= 2048 && jb == 8)
-        hipLaunchKernelGGL((bcgs_kernel<512, 4, 8>), dim3(ntiles), dim3(512),
-                0, s, tiles, which, ws, j0, d_ref, kmax, tau2, d_keep, first,
-                d_active);
-    else if(rows_max <= 4096 && jb == 4)
-        hipLaunchKernelGGL((bcgs_kernel<512, 8, 4>), dim3(ntiles), dim3(512),
-                0, s, tiles, which, ws, j0, d_ref, kmax, tau2, d_keep, first,
-                d_active);
+#define BCGS_LAUNCH(NT, RPT, PB) \
+        hipLaunchKernelGGL((bcgs_kernel<NT, RPT, PB>), dim3(ntiles), \
+                dim3(NT), 0, s, tiles, which, ws, j0, d_ref, kmax, tau2, \
+                d_keep, first, d_active)
+    if(jb == 4 && rows_max <= 1024)
+        BCGS_LAUNCH(256, 4, 4);
+    else if(jb == 4 && rows_max <= 2048)
+        BCGS_LAUNCH(256, 8, 4);
+    else if(jb == 4 && rows_max <= 4096)
+        BCGS_LAUNCH(512, 8, 4);
+    else if(jb == 8 && rows_max <= 1024)
+        BCGS_LAUNCH(256, 4, 8);
+    else if(jb == 8 && rows_max <= 2048)
+        BCGS_LAUNCH(512, 4, 8);
+#undef BCGS_LAUNCH
     else
         return STARSH_WRONG_PARAMETER;
     HIP_TRY(hipGetLastError());

@@ -75,6 +75,8 @@ int hip_mixed_set_first_bucket(int b);
 
 /* Column block of the generated-operand GEMMs (0 = by size, 64/128/256). */
 int starsh_hip_gen_gemm_bn(int bn);
+/* Panel width of the block Gram-Schmidt (8 or 4). */
+int starsh_hip_qr_panel(int width);
 size_t hip_driver_workspace_budget(void);
 double hip_driver_now(void);
 
