@@ -410,7 +410,7 @@ static size_t tile_bytes(int m, int n, int k, int maxrank)
 int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
         const STARSH_int *idx, const int *k, int int8, double tau,
         int *rank_out, int *flags_out, int *accepted, hip_accept_fn accept,
-        starsh_hip_stats *st)
+        double *S_out, int sstride, starsh_hip_stats *st)
 {
     if(ntiles == 0)
         return STARSH_SUCCESS;
@@ -424,9 +424,12 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
     RsddProblem *probs = malloc(sizeof(RsddProblem)*B->nprob);
     double **du = malloc(sizeof(double *)*ntiles);
     double **dv = malloc(sizeof(double *)*ntiles);
+    int64_t *so = malloc(sizeof(int64_t)*ntiles);
     if(!m || !n || !pp || !kc || !r0 || !c0 || !uo || !vo || !probs || !du ||
-            !dv)
+            !dv || !so)
         info = STARSH_MALLOC_ERROR;
+    for(STARSH_int t = 0; info == STARSH_SUCCESS && t < ntiles; t++)
+        so[t] = (int64_t)t*sstride;
     for(int q = 0; info == STARSH_SUCCESS && q < B->nprob; q++)
     {
         const hip_problem *hp = B->hp+q;
@@ -486,6 +489,11 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
         in.c0 = c0+b0;
         in.sketch_int8 = int8;
         in.tau = tau;
+        if(S_out)
+        {
+            in.S_out = S_out;
+            in.soff = so+b0;
+        }
         RsddOutput out;
         memset(&out, 0, sizeof(out));
         out.rank = rank_out+b0;
@@ -527,7 +535,7 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
         b0 = b1;
     }
     free(m); free(n); free(pp); free(kc); free(r0); free(c0); free(uo);
-    free(vo); free(probs); free(du); free(dv);
+    free(vo); free(probs); free(du); free(dv); free(so);
     return info;
 }
 
@@ -570,7 +578,7 @@ int hip_plain_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
         k[t] = B->maxrank[pidx[t]]+B->hp[pidx[t]].oversample;
     if(info == STARSH_SUCCESS)
         info = hip_driver_run(B, ntiles, pidx, idx, k, 0, 0.0, rk, fl, acc,
-                NULL, st);
+                NULL, NULL, 0, st);
     free(k); free(rk); free(fl); free(acc);
     return info;
 }

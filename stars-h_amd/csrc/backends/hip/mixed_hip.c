@@ -15,6 +15,7 @@
  *     fixed width), when the sketch exhausted the tile's numerical range, or
  *     at the plain width; otherwise the tile moves to the next bucket.
  * All pending tiles of a bucket go through the pipeline as one batch. */
+#include <math.h>
 #include "common.h"
 #include "rsdd.h"
 #include "hip_driver.h"
@@ -152,6 +153,32 @@ static int accept_mixed(int rank, int k, int kfull, int flags, int os)
     return (flags & 1) != 0;
 }
 
+/* Predicted rank of a tile from the singular values found with sketch
+ * width k (s[0..k): descending values, then the norm of the truncated rows,
+ * then zeros): the values away from the last `os` ones (which the sketch
+ * resolves least well) fix a geometric decay rate, and the tail is
+ * extrapolated until it drops below tol^2 times the energy, as in the rank
+ * rule of dsvfr.c:16-47.  Returns a large value for a decay too slow to
+ * extrapolate. */
+static double predicted_rank(const double *s, int k, int os, double tol)
+{
+    int p = 0;
+    double e = 0.0;
+    while(p < k && s[p] > 0.0)
+    {
+        e += s[p]*s[p];
+        p++;
+    }
+    const int hi = p-os-1, lo = hi/2;
+    if(hi < 4 || !(s[lo] > s[hi]))
+        return 1e9;
+    const double rho = pow(s[hi]/s[lo], 1.0/(hi-lo));
+    const double target = tol*sqrt(e*(1.0-rho*rho));
+    if(s[hi] <= target)
+        return hi;
+    return hi+log(target/s[hi])/log(rho);
+}
+
 int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
         const STARSH_int *idx, starsh_hip_stats *st)
 {
@@ -167,8 +194,23 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
     int *k = malloc(sizeof(int)*ntiles), *rk = malloc(sizeof(int)*ntiles);
     int *fl = malloc(sizeof(int)*ntiles), *acc = malloc(sizeof(int)*ntiles);
     int *first = malloc(sizeof(int)*B->nprob);
+    /* singular values of each tile of a round, NBMAX apart (the largest
+     * sketch width of any tile) */
+    int NBMAX = 1;
+    for(STARSH_int t = 0; t < ntiles; t++)
+    {
+        const int q = pidx[t];
+        const STARSH_blrf *F = B->F[q];
+        const int m = (int)F->row_cluster->size[F->block_far[2*idx[t]]];
+        const int n = (int)F->col_cluster->size[F->block_far[2*idx[t]+1]];
+        int kf = B->maxrank[q]+B->hp[q].oversample;
+        kf = kf < m ? kf : m;
+        kf = kf < n ? kf : n;
+        NBMAX = kf > NBMAX ? kf : NBMAX;
+    }
+    double *sv = malloc(sizeof(double)*(size_t)ntiles*NBMAX);
     if(!pend || !next || !bucket || !pq || !pi || !k || !rk || !fl || !acc ||
-            !first)
+            !first || !sv)
         info = STARSH_MALLOC_ERROR;
     /* Deflation threshold of the sketch columns: above the int8 sketch's
      * representation noise (2^-28 of a chunk's largest entry) and well
@@ -223,13 +265,25 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
             pi[u] = bi;
         }
         info = hip_driver_run(B, npend, pq, pi, k, int8, tau, rk, fl, acc,
-                accept_mixed, st);
+                accept_mixed, sv, NBMAX, st);
         STARSH_int nn = 0;
         for(STARSH_int u = 0; u < npend && info == STARSH_SUCCESS; u++)
             if(!acc[u])
             {
+                /* next bucket: at least one up, and wide enough for the
+                 * extrapolated rank r with max(oversample, r) spare
+                 * columns */
                 const STARSH_int t = pend[u];
-                bucket[t]++;
+                const int q = pq[u], os = B->hp[q].oversample;
+                double r = predicted_rank(sv+(size_t)u*NBMAX, k[u], os,
+                        B->tol[q]);
+                if(r < rk[u])
+                    r = rk[u];
+                const double want = r+(r > os ? r : os);
+                int b = bucket[t]+1;
+                while(b < NBUCKETS-1 && bucket_widths[b] < want)
+                    b++;
+                bucket[t] = b;
                 next[nn++] = t;
             }
         STARSH_int *tmp = pend;
@@ -240,7 +294,7 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
             last_pending_after_first = nn;
     }
     free(pend); free(next); free(bucket); free(pq); free(pi); free(k);
-    free(rk); free(fl); free(acc); free(first);
+    free(rk); free(fl); free(acc); free(first); free(sv);
     return info;
 }
 

@@ -27,6 +27,40 @@ __device__ inline double group_sum(double v, int g)
     return v;
 }
 
+// One-sided Jacobi step on the column pair (wa, wb) of length p by a group
+// of g threads (gl = index in the group): returns true if it rotated.
+__device__ inline bool rotate_pair(double *wa, double *wb, int p, int g,
+        int gl, double ctol)
+{
+    double al = 0.0, be = 0.0, ga = 0.0;
+    for(int i = gl; i < p; i += g)
+    {
+        double u = wa[i], v = wb[i];
+        al += u*u;
+        be += v*v;
+        ga += u*v;
+    }
+    al = group_sum(al, g);
+    be = group_sum(be, g);
+    ga = group_sum(ga, g);
+    if(!(al > 0.0 && be > 0.0 && fabs(ga) > ctol*sqrt(al*be)))
+        return false;
+    double zeta = (be-al)/(2.0*ga);
+    double t;
+    if(fabs(zeta) > 1e150)
+        t = 0.5/zeta;
+    else
+        t = (zeta >= 0.0 ? 1.0 : -1.0)/(fabs(zeta)+sqrt(1.0+zeta*zeta));
+    double c = 1.0/sqrt(1.0+t*t), s = c*t;
+    for(int i = gl; i < p; i += g)
+    {
+        double u = wa[i], v = wb[i];
+        wa[i] = c*u-s*v;
+        wb[i] = s*u+c*v;
+    }
+    return true;
+}
+
 // W: p x p working matrix (ld = ldw), in LDS or global memory.  A group
 // of g threads handles one column pair, g consecutive rows at a time.  In
 // LDS the groups are as small as needed to give every pair of a round its
@@ -68,37 +102,9 @@ __device__ void jacobi(double *W, int ldw, int p, bool in_lds,
                 }
                 if(b >= p)
                     continue;
-                double *wa = W+(int64_t)a*ldw, *wb = W+(int64_t)b*ldw;
-                double al = 0.0, be = 0.0, ga = 0.0;
-                for(int i = gl; i < p; i += g)
-                {
-                    double u = wa[i], v = wb[i];
-                    al += u*u;
-                    be += v*v;
-                    ga += u*v;
-                }
-                al = group_sum(al, g);
-                be = group_sum(be, g);
-                ga = group_sum(ga, g);
-                if(al > 0.0 && be > 0.0 && fabs(ga) > ctol*sqrt(al*be))
-                {
-                    double zeta = (be-al)/(2.0*ga);
-                    double t;
-                    if(fabs(zeta) > 1e150)
-                        t = 0.5/zeta;
-                    else
-                        t = (zeta >= 0.0 ? 1.0 : -1.0)/(fabs(zeta)+
-                                sqrt(1.0+zeta*zeta));
-                    double c = 1.0/sqrt(1.0+t*t), s = c*t;
-                    for(int i = gl; i < p; i += g)
-                    {
-                        double u = wa[i], v = wb[i];
-                        wa[i] = c*u-s*v;
-                        wb[i] = s*u+c*v;
-                    }
-                    if(gl == 0)
-                        atomicAdd(&rotations, 1);
-                }
+                if(rotate_pair(W+(int64_t)a*ldw, W+(int64_t)b*ldw, p, g, gl,
+                            ctol) && gl == 0)
+                    atomicAdd(&rotations, 1);
             }
             __syncthreads();
         }
@@ -109,18 +115,112 @@ __device__ void jacobi(double *W, int ldw, int p, bool in_lds,
     *sweeps_out = sweep;
 }
 
-// Dynamic LDS: sig[pmax] doubles, then (when lds_p > 0) W[lds_p^2]
-// doubles, then perm[pmax] ints.  With big == 0 the kernel takes the tiles
-// with p <= lds_p (W in LDS); with big != 0 those with p > LDS_PMAX (W in
-// their workspace, more threads to keep many column pairs in flight).
+// One-sided Jacobi for W (p x p, ld p) in global memory, one wave per
+// column pair.  The columns are split into blocks of BW; a sweep visits
+// every column pair once, in block-cyclic order: the pairs inside each
+// block, then for every block pair (I, J), I < J, the BW x BW cross pairs
+// in BW rounds of BW disjoint pairs.  The blocks being worked on sit in LDS
+// (L: 2 BW columns of p doubles), so each block pair costs one load and
+// one store of its columns instead of one pass over W per round.
+__device__ void jacobi_blocked(double *W, int p, double *L, int bw,
+        int *sweeps_out)
+{
+    __shared__ int rotations;
+    const int nb = (p+bw-1)/bw;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nwaves = blockDim.x >> 6;
+    const double ctol = (double)p*DBL_EPSILON;
+    auto width = [&](int blk) { return min(bw, p-blk*bw); };
+    auto load = [&](int blk, double *dst) {
+        const int cnt = width(blk)*p;
+        const double *src = W+(int64_t)blk*bw*p;
+        for(int e = threadIdx.x; e < cnt; e += blockDim.x)
+            dst[e] = src[e];
+    };
+    auto store = [&](int blk, const double *src) {
+        const int cnt = width(blk)*p;
+        double *dst = W+(int64_t)blk*bw*p;
+        for(int e = threadIdx.x; e < cnt; e += blockDim.x)
+            dst[e] = src[e];
+    };
+    int sweep;
+    for(sweep = 0; sweep < 60; sweep++)
+    {
+        if(threadIdx.x == 0)
+            rotations = 0;
+        __syncthreads();
+        for(int I = 0; I < nb; I++)
+        {
+            const int wi = width(I);
+            load(I, L);
+            __syncthreads();
+            // pairs inside block I (circle method)
+            const int we = wi+(wi & 1);
+            for(int round = 0; round < we-1; round++)
+            {
+                for(int pi = wave; pi < we/2; pi += nwaves)
+                {
+                    int x = pi, y = we-1-pi;
+                    int a = x == 0 ? 0 : 1+(x-1+round) % (we-1);
+                    int b = 1+(y-1+round) % (we-1);
+                    if(a > b)
+                    {
+                        int tmp = a;
+                        a = b;
+                        b = tmp;
+                    }
+                    if(b < wi && rotate_pair(L+(int64_t)a*p, L+(int64_t)b*p,
+                                p, 64, lane, ctol) && lane == 0)
+                        atomicAdd(&rotations, 1);
+                }
+                __syncthreads();
+            }
+            // cross pairs with every later block
+            double *LJ = L+(int64_t)bw*p;
+            for(int J = I+1; J < nb; J++)
+            {
+                const int wj = width(J);
+                load(J, LJ);
+                __syncthreads();
+                for(int round = 0; round < bw; round++)
+                {
+                    for(int a = wave; a < wi; a += nwaves)
+                    {
+                        const int b = (a+round) % bw;
+                        if(b < wj && rotate_pair(L+(int64_t)a*p,
+                                    LJ+(int64_t)b*p, p, 64, lane, ctol) &&
+                                lane == 0)
+                            atomicAdd(&rotations, 1);
+                    }
+                    __syncthreads();
+                }
+                store(J, LJ);
+                __syncthreads();
+            }
+            store(I, L);
+            __syncthreads();
+        }
+        if(rotations == 0)
+            break;
+        __syncthreads();
+    }
+    *sweeps_out = sweep;
+}
+
+// Dynamic LDS: sig[pmax] doubles, then the Jacobi buffer (W, lds_p^2
+// doubles, for the small tiles; two column blocks, 2 bw pmax doubles, for
+// the big ones), then perm[pmax] ints.  With big == 0 the kernel takes the
+// tiles with p <= lds_p (W in LDS); with big != 0 those with p > LDS_PMAX
+// (W in their workspace, blocked Jacobi through LDS).
 template <int NT>
 __global__ __launch_bounds__(NT) void core_kernel(TileDesc *tiles,
-        double *ws, int pmax, int lds_p, int big)
+        double *ws, int pmax, int lds_p, int big, int bw)
 {
     extern __shared__ double dyn[];
     double *sig = dyn;
     double *lds = dyn+pmax;
-    int *perm = (int *)(lds+(int64_t)lds_p*lds_p);
+    int *perm = (int *)(lds+(big ? (int64_t)2*bw*pmax :
+                (int64_t)lds_p*lds_p));
     __shared__ int s_rank;
     TileDesc &td = tiles[blockIdx.x];
     const int p = td.p, k = td.k, maxrank = td.maxrank;
@@ -157,7 +257,10 @@ __global__ __launch_bounds__(NT) void core_kernel(TileDesc *tiles,
     }
     __syncthreads();
     int sweeps;
-    jacobi<NT>(W, ldw, p, p <= lds_p, &sweeps);
+    if(big)
+        jacobi_blocked(W, p, lds, bw, &sweeps);
+    else
+        jacobi<NT>(W, ldw, p, true, &sweeps);
     __syncthreads();
     // Singular values: column norms of W.
     for(int j = threadIdx.x; j < p; j += blockDim.x)
@@ -245,23 +348,23 @@ int pipeline_core(TileDesc *d_tiles, int ntiles, double *ws, int pmax,
     const int lds_p = pm <= LDS_PMAX ? pm : LDS_PMAX;
     const size_t bytes = sizeof(double)*((size_t)pm+(size_t)lds_p*lds_p)+
         sizeof(int)*(size_t)pm;
-    static bool attr = false;
-    if(!attr)
-    {
-        HIP_TRY(hipFuncSetAttribute((const void *)core_kernel<CORE_THREADS>,
-                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                    (int)(sizeof(double)*(1024+LDS_PMAX*LDS_PMAX)+
-                        sizeof(int)*1024)));
-        attr = true;
-    }
+    HIP_TRY(hipFuncSetAttribute((const void *)core_kernel<CORE_THREADS>,
+                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     hipLaunchKernelGGL(core_kernel<CORE_THREADS>, dim3(ntiles),
-            dim3(CORE_THREADS), bytes, s, d_tiles, ws, pm, lds_p, 0);
+            dim3(CORE_THREADS), bytes, s, d_tiles, ws, pm, lds_p, 0, 0);
     HIP_TRY(hipGetLastError());
     if(pm > LDS_PMAX)
     {
-        const size_t bb = sizeof(double)*(size_t)pm+sizeof(int)*(size_t)pm;
+        // two column blocks of bw columns in LDS: 16 columns up to 576
+        // rows, 8 beyond (at most 160 KiB with sig and perm)
+        const int bw = pm <= 576 ? 16 : 8;
+        const size_t bb = sizeof(double)*((size_t)pm+(size_t)2*bw*pm)+
+            sizeof(int)*(size_t)pm;
+        HIP_TRY(hipFuncSetAttribute(
+                    (const void *)core_kernel<CORE_THREADS_BIG>,
+                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)bb));
         hipLaunchKernelGGL(core_kernel<CORE_THREADS_BIG>, dim3(ntiles),
-                dim3(CORE_THREADS_BIG), bb, s, d_tiles, ws, pm, 0, 1);
+                dim3(CORE_THREADS_BIG), bb, s, d_tiles, ws, pm, 0, 1, bw);
         HIP_TRY(hipGetLastError());
     }
     return STARSH_SUCCESS;

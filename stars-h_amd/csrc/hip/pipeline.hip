@@ -171,6 +171,62 @@ void dump(const char *name, const double *d_src, int64_t count, hipStream_t s)
 
 } // namespace
 
+namespace {
+
+// Gathers segments of a device buffer into one packed device buffer.
+__global__ __launch_bounds__(256) void pack_kernel(const double *src,
+        double *dst, const int64_t *seg, int nseg)
+{
+    // seg[3 s .. 3 s + 2] = source offset, packed offset, length
+    const int s = blockIdx.y;
+    if(s >= nseg)
+        return;
+    const int64_t so = seg[3*s], po = seg[3*s+1], len = seg[3*s+2];
+    for(int64_t e = (int64_t)blockIdx.x*blockDim.x+threadIdx.x; e < len;
+            e += (int64_t)gridDim.x*blockDim.x)
+        dst[po+e] = src[so+e];
+}
+
+
+// Copies the singular values (k per tile, from the workspace) to
+// S_out + soff[t], through one packed device-to-host transfer.
+int copy_segments(const double *d_ws, const std::vector<TileDesc> &h,
+        double *S_out, const int64_t *soff, hipStream_t s)
+{
+    const int nt = (int)h.size();
+    std::vector<int64_t> seg(3*(size_t)nt);
+    int64_t total = 0;
+    for(int t = 0; t < nt; t++)
+    {
+        seg[3*t] = h[t].s_off;
+        seg[3*t+1] = total;
+        seg[3*t+2] = h[t].k;
+        total += h[t].k;
+    }
+    TRY(pack_pool.reserve(sizeof(double)*total+sizeof(int64_t)*seg.size()));
+    TRY(stage_pool.reserve(sizeof(double)*total));
+    double *d_pack = (double *)pack_pool.p;
+    int64_t *d_seg = (int64_t *)(d_pack+total);
+    HIP_TRY(hipMemcpyAsync(d_seg, seg.data(), sizeof(int64_t)*seg.size(),
+                hipMemcpyHostToDevice, s));
+    for(int s0 = 0; s0 < nt; s0 += 65535)
+    {
+        const int cnt = std::min(nt-s0, 65535);
+        hipLaunchKernelGGL(pack_kernel, dim3(1, cnt), dim3(256), 0, s, d_ws,
+                d_pack, d_seg+3*(int64_t)s0, cnt);
+        HIP_TRY(hipGetLastError());
+    }
+    double *hs = (double *)stage_pool.p;
+    HIP_TRY(hipMemcpyAsync(hs, d_pack, sizeof(double)*total,
+                hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for(int t = 0; t < nt; t++)
+        memcpy(S_out+soff[t], hs+seg[3*t+1], sizeof(double)*h[t].k);
+    return STARSH_SUCCESS;
+}
+
+} // namespace
+
 // Runs the pipeline on ntiles tiles.  Either the kernel (kd, points) or
 // explicit tiles (d_a, a_off) define the tile contents.  Results: ranks in
 // rank_out, U/V in device memory (*d_uv, offsets in uoff/voff, allocated
@@ -393,9 +449,7 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     }
     HIP_TRY(hipEventRecord(ev[1], s));
     if(in->S_out)
-        for(int t = 0; t < nt; t++)
-            HIP_TRY(hipMemcpyAsync(in->S_out+in->soff[t], d_ws+h[t].s_off,
-                        sizeof(double)*h[t].k, hipMemcpyDeviceToHost, s));
+        TRY(copy_segments(d_ws, h, in->S_out, in->soff, s));
     HIP_TRY(hipStreamSynchronize(s));
     float ms = 0, ms_sk = 0;
     HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
@@ -415,23 +469,6 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     return STARSH_SUCCESS;
 }
 
-namespace {
-
-// Gathers segments of the factor buffer into one packed device buffer.
-__global__ __launch_bounds__(256) void pack_kernel(const double *src,
-        double *dst, const int64_t *seg, int nseg)
-{
-    // seg[3 s .. 3 s + 2] = source offset, packed offset, length
-    const int s = blockIdx.y;
-    if(s >= nseg)
-        return;
-    const int64_t so = seg[3*s], po = seg[3*s+1], len = seg[3*s+2];
-    for(int64_t e = (int64_t)blockIdx.x*blockDim.x+threadIdx.x; e < len;
-            e += (int64_t)gridDim.x*blockDim.x)
-        dst[po+e] = src[so+e];
-}
-
-} // namespace
 
 // Copies the factors of the tiles with dst_u[t] != NULL to host memory:
 // U_t (m[t] x rank[t]) to dst_u[t], V_t (n[t] x rank[t]) to dst_v[t].  The

@@ -51,11 +51,13 @@ typedef int (*hip_accept_fn)(int rank, int k, int kfull, int flags,
 /* Compresses far-field tiles idx[t] of problems pidx[t] (t < ntiles, all of
  * one kernel family and dimension) with sketch widths k[] in
  * workspace-sized chunks; accepted results (all when accept is NULL) go to
- * B->hf[pidx[t]] (rank and U/V copied to the host buffers). */
+ * B->hf[pidx[t]] (rank and U/V copied to the host buffers).  When S_out is
+ * not NULL the k[t] singular values of tile t (those of B, then the norm
+ * of the truncated rows, then zeros) go to S_out + t*sstride. */
 int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
         const STARSH_int *idx, const int *k, int int8, double tau,
         int *rank_out, int *flags_out, int *accepted, hip_accept_fn accept,
-        starsh_hip_stats *st);
+        double *S_out, int sstride, starsh_hip_stats *st);
 void hip_driver_rank_stats(const hip_batch *B, starsh_hip_stats *st);
 
 /* The plain and mixed forms over a list of tiles of a batch. */
