@@ -243,10 +243,28 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
     {
         pend[t] = t;
         bucket[t] = first[pidx[t]];
+        /* a diagonal tile of a symmetric kernel matrix holds each point's
+         * self-interaction and is nearly always of full numerical rank:
+         * it goes straight to the plain width */
+        const STARSH_blrf *F = B->F[pidx[t]];
+        if(F->block_far[2*idx[t]] == F->block_far[2*idx[t]+1] &&
+                F->row_cluster == F->col_cluster)
+            bucket[t] = NBUCKETS-1;
     }
     last_pending_after_first = npend;
     for(int round = 0; npend > 0 && info == STARSH_SUCCESS; round++)
     {
+        /* pending tiles grouped by bucket (stable), one batch per group so
+         * that every batch has one sketch width class */
+        STARSH_int cnt[NBUCKETS+1];
+        memset(cnt, 0, sizeof(cnt));
+        for(STARSH_int u = 0; u < npend; u++)
+            cnt[bucket[pend[u]]+1]++;
+        for(int b = 0; b < NBUCKETS; b++)
+            cnt[b+1] += cnt[b];
+        for(STARSH_int u = 0; u < npend; u++)
+            next[cnt[bucket[pend[u]]]++] = pend[u];
+        memcpy(pend, next, sizeof(STARSH_int)*npend);
         for(STARSH_int u = 0; u < npend; u++)
         {
             const STARSH_int t = pend[u];
@@ -264,8 +282,16 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
             pq[u] = q;
             pi[u] = bi;
         }
-        info = hip_driver_run(B, npend, pq, pi, k, int8, tau, rk, fl, acc,
-                accept_mixed, sv, NBMAX, st);
+        for(STARSH_int g0 = 0; g0 < npend && info == STARSH_SUCCESS; )
+        {
+            STARSH_int g1 = g0+1;
+            while(g1 < npend && bucket[pend[g1]] == bucket[pend[g0]])
+                g1++;
+            info = hip_driver_run(B, g1-g0, pq+g0, pi+g0, k+g0, int8, tau,
+                    rk+g0, fl+g0, acc+g0, accept_mixed,
+                    sv+(size_t)g0*NBMAX, NBMAX, st);
+            g0 = g1;
+        }
         STARSH_int nn = 0;
         for(STARSH_int u = 0; u < npend && info == STARSH_SUCCESS; u++)
             if(!acc[u])
