@@ -243,13 +243,6 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
     {
         pend[t] = t;
         bucket[t] = first[pidx[t]];
-        /* a diagonal tile of a symmetric kernel matrix holds each point's
-         * self-interaction and is nearly always of full numerical rank:
-         * it goes straight to the plain width */
-        const STARSH_blrf *F = B->F[pidx[t]];
-        if(F->block_far[2*idx[t]] == F->block_far[2*idx[t]+1] &&
-                F->row_cluster == F->col_cluster)
-            bucket[t] = NBUCKETS-1;
     }
     last_pending_after_first = npend;
     for(int round = 0; npend > 0 && info == STARSH_SUCCESS; round++)
