@@ -10,11 +10,23 @@ headline form (mixed when available, otherwise plain).  The JSON line also
 carries the API wall time (incl. factor copy-back), the dominant kernel's
 roofline, and a CPU baseline: the oracle (CPU restatement of the
 reference's OpenMP RSDD) timed on a bounded sample of tiles on the host
-cores and scaled to the 820 tiles.
+cores and scaled to all tiles.
+
+--config selects one of the other measured workloads (same JSON layout):
+  C3          3D Matern nu=1.5 beta 0.1, N=131072 uniform (srand 2), tile
+              2048, maxrank 512, oversample 16, eps 1e-7 (2080 tiles);
+  C4-coulomb  3D 1/r on 200 spheres r=0.02 x 500 points (srand 3),
+  C4-sin      3D sin(50 r)/r (diag 1.0) on the same points; N=100000,
+              tile 1536 (ragged 160), maxrank 384, oversample 16, eps 1e-6;
+  C5          48 2D exp problems (beta U[0.05,0.3], N log-uniform in
+              [2000, 40000], tile round64(sqrt(400 N)), numpy seed 4)
+              through starsh_blrm_approximate_batch, maxrank 384,
+              oversample 16, eps 1e-7.
 """
 import argparse
 import ctypes
 import json
+import math
 import os
 import sys
 import time
@@ -22,66 +34,162 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(ROOT, "stars-h_amd"), ROOT]
 
-N, NB, MAXRANK, OVERSAMPLE, TOL, BETA, SEED = 64000, 1600, 400, 16, 1e-7, \
-    0.1, 1
 PEAK_FP64 = 78.6  # TFLOP/s, MI355X dense FP64 (vector and matrix)
 
+# HBM bytes of one launch of the plain sketch kernel at C2, from the
+# FETCH_SIZE and WRITE_SIZE counters (rocprofv3 --pmc, one run per counter)
+# scaled by the same counters on a 1 GiB device copy (fetch x2.00, write
+# x1.00); commands and raw values in profiles/c2_sketch_hbm_bytes.md.
+C2_SKETCH_TRAFFIC = 7.346e9
 
-def run_form(S, backend, steps, warmup, autotune):
-    fn = S.init(backend, oversample=OVERSAMPLE)
-    p = S.TLRProblem("spatial", N, 2, NB, symm="S", seed=SEED,
-                     kernel="sqrexp", beta=BETA)
+WORKLOADS = {
+    "C2": dict(kind="spatial", kernel="sqrexp", N=64000, ndim=2, nb=1600,
+               maxrank=400, os=16, tol=1e-7, beta=0.1, nu=0.5, seed=1,
+               place="quasiuniform1",
+               desc="C2: 2D sqexp beta=0.1, N=64000, tile 1600, maxrank 400, "
+                    "oversample 16, eps 1e-7, symm S (820 tiles)",
+               metric="sec to TLR-compress N=64k sqexp at eps=1e-7: FP64 "
+                      "plain vs batched mixed-prec"),
+    "C3": dict(kind="spatial", kernel="matern", N=131072, ndim=3, nb=2048,
+               maxrank=512, os=16, tol=1e-7, beta=0.1, nu=1.5, seed=2,
+               place="rand",
+               desc="C3: 3D Matern nu=1.5 beta=0.1, N=131072 uniform, tile "
+                    "2048, maxrank 512, oversample 16, eps 1e-7, symm S "
+                    "(2080 tiles)",
+               metric="sec to TLR-compress N=131072 3D Matern at eps=1e-7"),
+    "C4-coulomb": dict(kind="coulomb", kernel="coulomb", N=100000, ndim=3,
+                       nb=1536, maxrank=384, os=16, tol=1e-6, seed=3,
+                       place="spheres",
+                       desc="C4: 3D Coulomb 1/r, 200 spheres x 500 points, "
+                            "tile 1536 (ragged 160), maxrank 384, oversample"
+                            " 16, eps 1e-6, symm S (2211 tiles)",
+                       metric="sec to TLR-compress N=100000 Coulomb at "
+                              "eps=1e-6"),
+    "C4-sin": dict(kind="sin", kernel="sin", N=100000, ndim=3, nb=1536,
+                   maxrank=384, os=16, tol=1e-6, seed=3, place="spheres",
+                   wave_k=50.0, diag=1.0,
+                   desc="C4: 3D sin(50r)/r, 200 spheres x 500 points, tile "
+                        "1536 (ragged 160), maxrank 384, oversample 16, eps "
+                        "1e-6, symm S (2211 tiles)",
+                   metric="sec to TLR-compress N=100000 sin(kr)/r at "
+                          "eps=1e-6"),
+    "C5": dict(kind="batch", kernel="exp", ndim=2, maxrank=384, os=16,
+               tol=1e-7, seed=4,
+               desc="C5: 48 2D exp problems, beta U[0.05,0.3], N log-uniform"
+                    " in [2000, 40000], tile round64(sqrt(400N)), maxrank "
+                    "384, oversample 16, eps 1e-7, one batch call",
+               metric="sec to TLR-compress 48 2D exp problems in one batch "
+                      "call at eps=1e-7"),
+}
+
+
+def c5_specs(seed):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(48):
+        beta = float(rng.uniform(0.05, 0.3))
+        N = int(round(math.exp(rng.uniform(math.log(2000),
+                                           math.log(40000)))))
+        out.append((N, int(64 * round(math.sqrt(400 * N) / 64)), beta))
+    return out
+
+
+def make_problems(S, w):
+    """TLRProblem objects of a workload (a list; one except for C5)."""
+    if w["kind"] == "batch":
+        return [S.TLRProblem("spatial", N, 2, nb, symm="S", seed=100 + q,
+                             kernel="exp", beta=beta)
+                for q, (N, nb, beta) in enumerate(c5_specs(w["seed"]))]
+    if w["place"] == "spheres":
+        pts = S.spheres(200, 500, 0.02, w["seed"])
+        return [S.TLRProblem(w["kind"], pts.shape[1], 3, w["nb"], symm="S",
+                             points=pts, wave_k=w.get("wave_k", 1.0),
+                             diag=w.get("diag", 1.0))]
+    place = {"quasiuniform1": S.STARSH_PARTICLES_QUASIUNIFORM1,
+             "rand": S.STARSH_PARTICLES_RAND}[w["place"]]
+    return [S.TLRProblem("spatial", w["N"], w["ndim"], w["nb"], symm="S",
+                         seed=w["seed"], place=place, kernel=w["kernel"],
+                         beta=w["beta"], nu=w["nu"])]
+
+
+def compress(S, ps, w, fn):
+    if w["kind"] != "batch":
+        return ps[0].approximate(w["maxrank"], w["tol"], 0, fn)
+    return S.approximate_batch(ps, w["maxrank"], w["tol"])
+
+
+def run_form(S, w, backend, steps, warmup, autotune):
+    fn = S.init(backend, oversample=w["os"])
+    ps = make_problems(S, w)
     schedule = None
-    if backend == "HIP_MIXED" and autotune:
+    if backend == "HIP_MIXED" and autotune and w["kind"] != "batch":
         # plan step (outside the timed steps): first rank bucket by trial
-        first, secs = p.autotune(MAXRANK, TOL)
+        first, secs = ps[0].autotune(w["maxrank"], w["tol"])
         schedule = {"first_bucket_width": [32, 64, 128, 256, 512][first],
                     "candidate_device_seconds": [round(x, 6) for x in secs]}
-    dev, wall, dom, launches = [], [], 0.0, 0
+    dev, wall, dom = [], [], []
     for it in range(warmup + steps):
-        info = p.approximate(MAXRANK, TOL, 0, fn)
+        if it > 0:
+            # a compression rewrites the format when it finds false
+            # far-field tiles; every step starts from a fresh format
+            ps = make_problems(S, w)
+        info = compress(S, ps, w, fn)
         assert info == 0, info
         st = S.stats()
         if it >= warmup:
             dev.append(st.device_seconds)
             wall.append(st.wall_seconds)
-            dom += st.dominant_kernel_seconds
-            launches += st.dominant_kernel_launches
+            dom.append(st.dominant_kernel_seconds)
         if it < warmup + steps - 1:
-            p.free_matrix()
-    err, terr, tnorm = p.dfe_tiles()
-    ranks = p.ranks()
+            for p in ps:
+                p.free_matrix()
+    errs, tmax, ranks = [], 0.0, []
+    for p in ps:
+        e, terr, _ = p.dfe_tiles()
+        errs.append(e)
+        if len(terr):
+            tmax = max(tmax, float(terr.max()))
+        ranks.extend(p.ranks().tolist())
+    t = sum(dev) / len(dev)
     res = {
-        "device_seconds": sum(dev) / len(dev),
+        "device_seconds": t,
         "api_wall_seconds": sum(wall) / len(wall),
-        "dominant_kernel_seconds": dom / max(launches, 1),
+        # summed over the launches of one step
+        "dominant_kernel_seconds": sum(dom) / len(dom),
+        "dominant_kernel_launches": int(st.dominant_kernel_launches),
         "sketch_flops": st.sketch_bytes_or_flops,
         "algorithmic_flops": st.flops_algorithmic,
         "tiles": int(st.ntiles),
+        "tiles_per_second": st.ntiles / t if t > 0 else 0.0,
         "false_far": int(st.nfalse_far),
-        "rank_min": int(ranks.min()), "rank_max": int(ranks.max()),
-        "rank_avg": float(ranks.mean()),
-        "global_rel_error": err,
-        "max_tile_rel_error": float(terr.max()),
-        "points": p.points(),
-        "cluster": p.cluster_ranges(),
-        "tiles_list": p.tiles(),
+        "rank_min": int(min(ranks)) if ranks else 0,
+        "rank_max": int(max(ranks)) if ranks else 0,
+        "rank_avg": float(sum(ranks) / len(ranks)) if ranks else 0.0,
+        "global_rel_error": max(errs),
+        "max_tile_rel_error": tmax,
+        "problems": ps,
     }
     if schedule is not None:
         res["schedule"] = schedule
-    p.free_matrix()
     return res
 
 
-def cpu_baseline(points, cluster, tiles, nsample):
+def cpu_baseline(w, p, nsample):
+    """The oracle (OpenMP restatement of the reference tile loop) on a
+    spread sample of the tiles, scaled to all tiles."""
     from oracle import oracle as O
     nthreads = min(16, os.cpu_count() or 1)
-    kd = O.kernel("sqrexp", 2, beta=BETA)
+    kd = O.kernel(w["kernel"], w["ndim"], beta=w.get("beta", 0.1),
+                  nu=w.get("nu", 0.5), wave_k=w.get("wave_k", 1.0),
+                  diag=w.get("diag", 1.0))
+    tiles = p.tiles()
     step = max(1, len(tiles) // nsample)
     sample = tiles[::step][:nsample]
+    st, sz = p.cluster_ranges()
     t0 = time.perf_counter()
-    O.compress(kd, points, cluster[0], cluster[1], sample, MAXRANK,
-               OVERSAMPLE, TOL, nthreads=nthreads, want_factors=False)
+    O.compress(kd, p.points(), st, sz, sample, w["maxrank"], w["os"],
+               w["tol"], nthreads=nthreads, want_factors=False)
     dt = time.perf_counter() - t0
     return {"value": dt * len(tiles) / len(sample), "unit": "s",
             "cores": nthreads, "kind": "port",
@@ -96,6 +204,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="C2", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-sample", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="plain,mixed")
@@ -125,6 +234,7 @@ def main():
         print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n}))
         return
     import starsh_hip as S
+    w = WORKLOADS[args.config]
     for kv in args.set:
         name, val = kv.split("=")
         assert S.lib().starsh_hip_set_option(
@@ -135,21 +245,17 @@ def main():
     forms = {}
     for form in args.forms.split(","):
         backend = {"plain": "HIP", "mixed": "HIP_MIXED"}[form]
-        try:
-            print(f"bench: {form} form, {args.warmup} warmup + {args.steps} "
-                  f"steps", file=sys.stderr, flush=True)
-            forms[form] = run_form(S, backend, args.steps, args.warmup,
-                                   not args.no_autotune)
-        except AssertionError as e:
-            print(f"bench: {form} form unavailable ({e})", file=sys.stderr)
+        print(f"bench: {args.config} {form} form, {args.warmup} warmup + "
+              f"{args.steps} steps", file=sys.stderr, flush=True)
+        forms[form] = run_form(S, w, backend, args.steps, args.warmup,
+                               not args.no_autotune)
     plain = forms.get("plain")
     head = forms.get("mixed") or plain
     head_name = "mixed" if "mixed" in forms else "plain"
     cpu = None
-    if not args.no_cpu and plain is not None:
+    if not args.no_cpu and plain is not None and w["kind"] != "batch":
         print("bench: CPU baseline sample", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(plain["points"], plain["cluster"],
-                           plain["tiles_list"], args.cpu_sample)
+        cpu = cpu_baseline(w, plain["problems"][0], args.cpu_sample)
     dom = plain or head
     dom_t = dom["dominant_kernel_seconds"]
     achieved = dom["sketch_flops"] / dom_t / 1e12 if dom_t > 0 else 0.0
@@ -157,11 +263,9 @@ def main():
     for name, r in forms.items():
         summary[name] = {k: (float(f"{v:.6g}") if isinstance(v, float)
                              else v)
-                         for k, v in r.items()
-                         if k not in ("points", "cluster", "tiles_list")}
+                         for k, v in r.items() if k != "problems"}
     line = {
-        "metric": "sec to TLR-compress N=64k sqexp at eps=1e-7: FP64 plain "
-                  "vs batched mixed-prec",
+        "metric": w["metric"],
         "value": round(head["device_seconds"], 6),
         "unit": "s",
         "n_gpus": 1,
@@ -172,11 +276,8 @@ def main():
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": "f64" if head_name == "plain" else "i8+f64",
-        "data": "synthetic (seeded jittered grid, glibc rand)",
-        "config": {"workload": "C2: 2D sqexp beta=0.1, N=64000, tile 1600, "
-                               "maxrank 400, oversample 16, eps 1e-7, "
-                               "symm S (820 tiles)",
-                   "form": head_name},
+        "data": "synthetic (seeded point sets, glibc rand / numpy)",
+        "config": {"workload": w["desc"], "form": head_name},
         "forms": summary,
         "speedup_mixed_vs_plain": (round(plain["device_seconds"] /
                                          forms["mixed"]["device_seconds"], 3)
@@ -187,9 +288,13 @@ def main():
                      "bound": "mfma", "achieved": round(achieved, 3),
                      "peak": PEAK_FP64, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64, 4),
-                     "traffic": None},
+                     "traffic": (C2_SKETCH_TRAFFIC if args.config == "C2"
+                                 and plain is not None else None)},
         "cpu_baseline": cpu,
     }
+    for r in forms.values():
+        for p in r["problems"]:
+            p.free_matrix()
     print(json.dumps(line))
 
 
