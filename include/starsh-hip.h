@@ -98,6 +98,8 @@ int starsh_hip_get_stats(starsh_hip_stats *stats);
  *                         autotuner's schedule table (0 without an entry);
  *   "gemm_bn"             column block of the generated-tile GEMMs: 0 (by
  *                         size, default), 64, 128 or 256;
+ *   "gemm_rw"             16-row groups per wave of those GEMMs: 0 (by
+ *                         size, default), 1 or 2;
  *   "qr_panel"            panel width of the block Gram-Schmidt, 4
  *                         (default) or 8 (tiles of at most 2048 rows);
  *   "release_workspace"   (value ignored) frees the device workspace and

@@ -49,6 +49,8 @@ int starsh_hip_set_option(const char *name, double value)
         return hip_mixed_set_first_bucket((int)value);
     if(!strcmp(name, "gemm_bn"))
         return starsh_hip_gen_gemm_bn((int)value);
+    if(!strcmp(name, "gemm_rw"))
+        return starsh_hip_gen_gemm_rw((int)value);
     if(!strcmp(name, "qr_panel"))
         return starsh_hip_qr_panel((int)value);
     if(!strcmp(name, "release_workspace"))

@@ -15,8 +15,10 @@ namespace {
 constexpr int BM = 64;
 constexpr int BK = 16;
 
-// Column block width (0: by problem size); set by starsh_hip_set_option.
+// Column block width and 16-row groups per wave (1 or 2), 0 for the
+// choice by size below; set by starsh_hip_set_option.
 int gen_gemm_bn = 0;
+int gen_gemm_rw = 0;
 
 // Workgroups are dispatched to the 8 XCDs of the chip round robin (group
 // L runs on XCD L % 8), and each XCD has its own L2.  The 1-D grid is
@@ -65,11 +67,12 @@ inline int64_t mapped_grid(int ntiles, int nbm, int nbn, bool shared_b)
 // PROJ=false: rows are row points, K runs over column points.
 // PROJ=true : rows are column points, K runs over row points (A^T).
 // gnbm x gnbn is the block grid of the largest tile of the launch.
-template <int NDIM, int FAM, int BN, bool PROJ>
+template <int NDIM, int FAM, int BN, int RW, bool PROJ>
 __global__ __launch_bounds__(256) void gen_gemm_kernel(const TileDesc *tiles,
         GemmArgs g, int ntiles, int gnbm, int gnbn)
 {
     constexpr int NF = BN/16;
+    constexpr int BMR = BM*RW;      // rows per block: RW 16-row groups/wave
     int tile, bm, bn;
     if(!map_block(blockIdx.x, ntiles, gnbm, gnbn, !PROJ, tile, bm, bn))
         return;
@@ -100,33 +103,41 @@ __global__ __launch_bounds__(256) void gen_gemm_kernel(const TileDesc *tiles,
     }
     if(M <= 0 || N <= 0)
         return;
-    if(bm*BM >= M || bn*BN >= N)
+    if(bm*BMR >= M || bn*BN >= N)
         return;
-    const int row0 = bm*BM, col0 = bn*BN;
+    const int row0 = bm*BMR, col0 = bn*BN;
     // 16-column fragments that hold columns of this tile (wave-uniform)
     const int nf_live = (N-col0+15)/16 < NF ? (N-col0+15)/16 : NF;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
     __shared__ double Bs[2][BK][BN+2];
 
-    // this lane's generated row (fixed) and its K offset inside a 4-step
-    const int grow = row0+wave*16+(lane & 15);
-    const bool row_ok = grow < M;
-    double xi[NDIM];
+    // this lane's generated rows (fixed) and its K offset inside a 4-step
+    const int grow0 = row0+wave*16*RW+(lane & 15);
+    unsigned okmask = 0;      // bit r: row grow0 + 16 r is inside the tile
+    double xi[RW][NDIM];
 #pragma unroll
-    for(int d = 0; d < NDIM; d++)
-        xi[d] = row_ok ? xrow[d*nrow+rbase+grow] : 0.0;
+    for(int r = 0; r < RW; r++)
+    {
+        const bool ok = grow0+16*r < M;
+        okmask |= ok ? 1u << r : 0u;
+#pragma unroll
+        for(int d = 0; d < NDIM; d++)
+            xi[r][d] = ok ? xrow[d*nrow+rbase+grow0+16*r] : 0.0;
+    }
     const int kl = lane >> 4;
 
-    double4_t acc[NF];
+    double4_t acc[RW][NF];
 #pragma unroll
-    for(int f = 0; f < NF; f++)
-        acc[f] = double4_t{0.0, 0.0, 0.0, 0.0};
+    for(int r = 0; r < RW; r++)
+#pragma unroll
+        for(int f = 0; f < NF; f++)
+            acc[r][f] = double4_t{0.0, 0.0, 0.0, 0.0};
 
     // B staging: thread loads (kk = tid%16, j = tid/16 + 16 s)
     const int bk = tid & 15, bj = tid >> 4;
     double breg[NF];
-    double areg[4];
+    double areg[RW][4];
 
     auto load_b = [&](int k0) {
 #pragma unroll
@@ -142,7 +153,7 @@ __global__ __launch_bounds__(256) void gen_gemm_kernel(const TileDesc *tiles,
             Bs[buf][bk][bj+16*s] = breg[s];
     };
     auto gen_a = [&](int k0) {
-        if constexpr(BN <= 128)
+        if constexpr(BN*RW <= 128)
         {
             // few MFMAs per entry: issue the point loads together
             double yv[4][NDIM];
@@ -156,12 +167,14 @@ __global__ __launch_bounds__(256) void gen_gemm_kernel(const TileDesc *tiles,
                     yv[q][d] = xk[d*nk+j];
             }
 #pragma unroll
-            for(int q = 0; q < 4; q++)
-            {
-                const int k = k0+4*q+kl;
-                const double v = kval_pt<NDIM, FAM>(t.kp, xi, yv[q]);
-                areg[q] = (row_ok && k < K) ? v : 0.0;
-            }
+            for(int r = 0; r < RW; r++)
+#pragma unroll
+                for(int q = 0; q < 4; q++)
+                {
+                    const int k = k0+4*q+kl;
+                    const double v = kval_pt<NDIM, FAM>(t.kp, xi[r], yv[q]);
+                    areg[r][q] = ((okmask >> r & 1u) && k < K) ? v : 0.0;
+                }
         }
         else
         {
@@ -170,9 +183,13 @@ __global__ __launch_bounds__(256) void gen_gemm_kernel(const TileDesc *tiles,
             for(int q = 0; q < 4; q++)
             {
                 const int k = k0+4*q+kl;
-                const double v = kval<NDIM, FAM>(t.kp, xi, xk, nk,
-                        kbase+(k < K ? k : K-1));
-                areg[q] = (row_ok && k < K) ? v : 0.0;
+#pragma unroll
+                for(int r = 0; r < RW; r++)
+                {
+                    const double v = kval<NDIM, FAM>(t.kp, xi[r], xk, nk,
+                            kbase+(k < K ? k : K-1));
+                    areg[r][q] = ((okmask >> r & 1u) && k < K) ? v : 0.0;
+                }
             }
         }
     };
@@ -184,17 +201,19 @@ __global__ __launch_bounds__(256) void gen_gemm_kernel(const TileDesc *tiles,
     int buf = 0;
     for(int k0 = 0; k0 < K; k0 += BK)
     {
-        double acur[4];
+        double acur[RW][4];
 #pragma unroll
-        for(int q = 0; q < 4; q++)
-            acur[q] = areg[q];
+        for(int r = 0; r < RW; r++)
+#pragma unroll
+            for(int q = 0; q < 4; q++)
+                acur[r][q] = areg[r][q];
         const bool more = k0+BK < K;
         if(more)
         {
             load_b(k0+BK);
             gen_a(k0+BK);
         }
-        if constexpr(BN <= 128)
+        if constexpr(BN*RW <= 128)
         {
             // fragments past the tile's own width are skipped (uniform)
 #pragma unroll
@@ -205,8 +224,10 @@ __global__ __launch_bounds__(256) void gen_gemm_kernel(const TileDesc *tiles,
                     for(int q = 0; q < 4; q++)
                     {
                         const double b = Bs[buf][4*q+kl][f*16+(lane & 15)];
-                        acc[f] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                                acur[q], b, acc[f], 0, 0, 0);
+#pragma unroll
+                        for(int r = 0; r < RW; r++)
+                            acc[r][f] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                    acur[r][q], b, acc[r][f], 0, 0, 0);
                     }
                 }
         }
@@ -218,9 +239,12 @@ __global__ __launch_bounds__(256) void gen_gemm_kernel(const TileDesc *tiles,
 #pragma unroll
                 for(int f = 0; f < NF; f++)
                 {
+                    // each B value feeds RW MFMAs
                     const double b = Bs[buf][4*q+kl][f*16+(lane & 15)];
-                    acc[f] = __builtin_amdgcn_mfma_f64_16x16x4f64(acur[q], b,
-                            acc[f], 0, 0, 0);
+#pragma unroll
+                    for(int r = 0; r < RW; r++)
+                        acc[r][f] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                acur[r][q], b, acc[r][f], 0, 0, 0);
                 }
             }
         }
@@ -231,46 +255,73 @@ __global__ __launch_bounds__(256) void gen_gemm_kernel(const TileDesc *tiles,
     }
     // epilogue: lane holds rows lane/16 + 4 v, column lane%16 of each frag
 #pragma unroll
-    for(int f = 0; f < NF; f++)
-    {
-        const int gj = col0+f*16+(lane & 15);
-        if(gj >= N)
-            continue;
+    for(int r = 0; r < RW; r++)
 #pragma unroll
-        for(int v = 0; v < 4; v++)
+        for(int f = 0; f < NF; f++)
         {
-            const int gi = row0+wave*16+(lane >> 4)+4*v;
-            if(gi < M)
-                C[(int64_t)gj*ldc+gi] = acc[f][v];
+            const int gj = col0+f*16+(lane & 15);
+            if(gj >= N)
+                continue;
+#pragma unroll
+            for(int v = 0; v < 4; v++)
+            {
+                const int gi = row0+wave*16*RW+r*16+(lane >> 4)+4*v;
+                if(gi < M)
+                    C[(int64_t)gj*ldc+gi] = acc[r][f][v];
+            }
         }
-    }
+}
+
+template <int NDIM, int FAM, int BN, int RW, bool PROJ>
+int launch_one(const TileDesc *d_tiles, int ntiles, int mmax, int nmax,
+        const GemmArgs &g, hipStream_t s)
+{
+    const int nbm = (mmax+BM*RW-1)/(BM*RW);
+    const int nbn = (nmax+BN-1)/BN;
+    const int64_t grid = mapped_grid(ntiles, nbm, nbn, !PROJ);
+    if(grid > 0x7fffffff)
+        return STARSH_WRONG_PARAMETER;
+    hipLaunchKernelGGL((gen_gemm_kernel<NDIM, FAM, BN, RW, PROJ>),
+            dim3((unsigned)grid), dim3(256), 0, s, d_tiles, g, ntiles, nbm,
+            nbn);
+    HIP_TRY(hipGetLastError());
+    return STARSH_SUCCESS;
 }
 
 template <int NDIM, int FAM, bool PROJ>
 int launch_nd(const TileDesc *d_tiles, int ntiles, int mmax, int nmax,
         const GemmArgs &g, hipStream_t s)
 {
-    const int nbm = (mmax+BM-1)/BM;
-    int bn = gen_gemm_bn > 0 ? gen_gemm_bn : (nmax > 128 ? 256 :
-            (nmax > 64 ? 128 : 64));
-    const int nbn = (nmax+bn-1)/bn;
-    const int64_t grid = mapped_grid(ntiles, nbm, nbn, !PROJ);
-    if(grid > 0x7fffffff)
-        return STARSH_WRONG_PARAMETER;
-    if(bn == 256)
-        hipLaunchKernelGGL((gen_gemm_kernel<NDIM, FAM, 256, PROJ>),
-                dim3((unsigned)grid), dim3(256), 0, s, d_tiles, g, ntiles,
-                nbm, nbn);
-    else if(bn == 128)
-        hipLaunchKernelGGL((gen_gemm_kernel<NDIM, FAM, 128, PROJ>),
-                dim3((unsigned)grid), dim3(256), 0, s, d_tiles, g, ntiles,
-                nbm, nbn);
+    // By size (measured on MI355X at tile 1600): the sketch's wide
+    // column blocks reuse each generated entry most; the projection (N =
+    // kept columns, often just under a multiple of 256) wastes less on
+    // 128-column blocks with two 16-row groups per wave.
+    int bn, rw;
+    if(PROJ)
+    {
+        bn = nmax > 64 ? 128 : 64;
+        rw = nmax > 64 ? 2 : 1;
+    }
     else
-        hipLaunchKernelGGL((gen_gemm_kernel<NDIM, FAM, 64, PROJ>),
-                dim3((unsigned)grid), dim3(256), 0, s, d_tiles, g, ntiles,
-                nbm, nbn);
-    HIP_TRY(hipGetLastError());
-    return STARSH_SUCCESS;
+    {
+        bn = nmax > 128 ? 256 : (nmax > 64 ? 128 : 64);
+        rw = 1;
+    }
+    if(gen_gemm_bn > 0)
+        bn = gen_gemm_bn;
+    if(gen_gemm_rw > 0)
+        rw = gen_gemm_rw;
+    if(bn == 256)
+        return rw == 2 ? launch_one<NDIM, FAM, 256, 2, PROJ>(d_tiles, ntiles,
+                mmax, nmax, g, s) : launch_one<NDIM, FAM, 256, 1, PROJ>(
+                d_tiles, ntiles, mmax, nmax, g, s);
+    if(bn == 128)
+        return rw == 2 ? launch_one<NDIM, FAM, 128, 2, PROJ>(d_tiles, ntiles,
+                mmax, nmax, g, s) : launch_one<NDIM, FAM, 128, 1, PROJ>(
+                d_tiles, ntiles, mmax, nmax, g, s);
+    return rw == 2 ? launch_one<NDIM, FAM, 64, 2, PROJ>(d_tiles, ntiles,
+            mmax, nmax, g, s) : launch_one<NDIM, FAM, 64, 1, PROJ>(d_tiles,
+            ntiles, mmax, nmax, g, s);
 }
 
 template <bool PROJ>
@@ -287,6 +338,14 @@ struct GenLaunch
 };
 
 } // namespace
+
+extern "C" int starsh_hip_gen_gemm_rw(int rw)
+{
+    if(rw != 0 && rw != 1 && rw != 2)
+        return STARSH_WRONG_PARAMETER;
+    gen_gemm_rw = rw;
+    return STARSH_SUCCESS;
+}
 
 extern "C" int starsh_hip_gen_gemm_bn(int bn)
 {

@@ -77,6 +77,8 @@ int hip_mixed_set_first_bucket(int b);
 
 /* Column block of the generated-operand GEMMs (0 = by size, 64/128/256). */
 int starsh_hip_gen_gemm_bn(int bn);
+/* 16-row groups per wave of the generated-tile GEMMs (0 = by size, 1, 2). */
+int starsh_hip_gen_gemm_rw(int rw);
 /* Panel width of the block Gram-Schmidt (8 or 4). */
 int starsh_hip_qr_panel(int width);
 size_t hip_driver_workspace_budget(void);
