@@ -165,6 +165,38 @@ __device__ inline double kernel_value(const KernelParams &kp,
 // allowed.  It agrees with kernel_value to a few units in the last place;
 // tiles handed back to the caller (near field, error check) use
 // kernel_value.
+template <int FAM>
+__device__ inline double kval_of_dist(const KernelParams &kp, double dist)
+{
+    switch(FAM)
+    {
+        case STARSH_HIP_EXP:
+            return dist == 0 ? kp.sigma+kp.noise :
+                kp.sigma*exp(sqrt(dist)*kp.ibexp);
+        case STARSH_HIP_SQREXP:
+            return dist == 0 ? kp.sigma+kp.noise :
+                kp.sigma*exp(dist*kp.ibsqr);
+        case STARSH_HIP_MATERN:
+            return dist == 0 ? kp.sigma+kp.noise :
+                kp.sigma*matern_closed(kp.nu, sqrt(dist)*kp.theta);
+        case STARSH_HIP_MATERN2:
+            return dist == 0 ? kp.sigma+kp.noise :
+                kp.sigma*matern_closed(kp.nu, sqrt(dist)*kp.ibeta);
+        case STARSH_HIP_COULOMB:
+            return dist == 0 ? 0.0 : rsqrt(dist);
+        case STARSH_HIP_SIN:
+        {
+            const double ri = rsqrt(dist);
+            return dist == 0 ? kp.diag : sin(kp.wave_k*(dist*ri))*ri;
+        }
+        default:
+        {
+            const double ri = rsqrt(dist);
+            return dist == 0 ? kp.diag : cos(kp.wave_k*(dist*ri))*ri;
+        }
+    }
+}
+
 template <int NDIM, int FAM>
 __device__ inline double kval_pt(const KernelParams &kp,
         const double (&xi)[NDIM], const double (&yj)[NDIM])
@@ -176,41 +208,75 @@ __device__ inline double kval_pt(const KernelParams &kp,
         const double t = xi[d]-yj[d];
         dist = fma(t, t, dist);
     }
+    return kval_of_dist<FAM>(kp, dist);
+}
+
+// exp(x) for finite x <= 0 to within about two units in the last place, by
+// the table-driven method of P. T. P. Tang ("Table-driven implementation of
+// the exponential function in IEEE floating-point arithmetic", ACM TOMS
+// 15(2), 1989): x = (32 N + j) ln2/32 + r with |r| <= ln2/64, so exp(x) =
+// 2^N 2^(j/32) e^r, with e^r - 1 from its Taylor polynomial of degree 6
+// (remainder below 2^-58).  tab[j] = 2^(j/32), j = 0..31 (exp2_table).
+// Arguments below -1000 give 0, like exp.
+__device__ inline double exp_nonpos(double x, const double *tab)
+{
+    x = x > -1000.0 ? x : -1000.0;
+    const double nd = rint(x*46.16624130844683);        // 32/ln2
+    const int n = (int)nd;
+    double r = fma(-nd, 0.02166084939249829, x);        // ln2/32, high part
+    r = fma(-nd, 7.247021293269686e-19, r);             // low part
+    double p = fma(r, 1.0/720.0, 1.0/120.0);
+    p = fma(p, r, 1.0/24.0);
+    p = fma(p, r, 1.0/6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p *= r;
+    const double t = tab[n & 31];
+    return ldexp(fma(t, p, t), n >> 5);
+}
+
+// Fills tab[0..31] with 2^(j/32); called by the first 32 threads.
+__device__ inline void exp2_table(double *tab, int tid)
+{
+    if(tid < 32)
+        tab[tid] = exp2((double)tid*(1.0/32.0));
+}
+
+// kval_of_dist with exp_nonpos for the exponential families (exp, sqexp,
+// Matern); the other families are unchanged.
+template <int FAM>
+__device__ inline double kval_of_dist_tab(const KernelParams &kp, double dist,
+        const double *tab)
+{
+    // computed for every distance and selected afterwards, so that the
+    // compiler keeps the evaluation free of branches
     double v;
     switch(FAM)
     {
         case STARSH_HIP_EXP:
-            v = dist == 0 ? kp.sigma+kp.noise :
-                kp.sigma*exp(sqrt(dist)*kp.ibexp);
+            v = kp.sigma*exp_nonpos(sqrt(dist)*kp.ibexp, tab);
             break;
         case STARSH_HIP_SQREXP:
-            v = dist == 0 ? kp.sigma+kp.noise : kp.sigma*exp(dist*kp.ibsqr);
+            v = kp.sigma*exp_nonpos(dist*kp.ibsqr, tab);
             break;
         case STARSH_HIP_MATERN:
-            v = dist == 0 ? kp.sigma+kp.noise :
-                kp.sigma*matern_closed(kp.nu, sqrt(dist)*kp.theta);
-            break;
         case STARSH_HIP_MATERN2:
-            v = dist == 0 ? kp.sigma+kp.noise :
-                kp.sigma*matern_closed(kp.nu, sqrt(dist)*kp.ibeta);
-            break;
-        case STARSH_HIP_COULOMB:
-            v = dist == 0 ? 0.0 : rsqrt(dist);
-            break;
-        case STARSH_HIP_SIN:
         {
-            const double ri = rsqrt(dist);
-            v = dist == 0 ? kp.diag : sin(kp.wave_k*(dist*ri))*ri;
+            const double x = sqrt(dist)*(FAM == STARSH_HIP_MATERN ?
+                    kp.theta : kp.ibeta);
+            const double e = exp_nonpos(-x, tab);
+            // 1, 1 + x or 1 + x + x^2/3 for nu = 0.5, 1.5, 2.5, as one
+            // polynomial whose coefficients do not depend on the entry
+            const double c1 = kp.nu == 0.5 ? 0.0 : 1.0;
+            const double c2 = kp.nu == 0.5 || kp.nu == 1.5 ? 0.0 : 1.0/3.0;
+            const double f = fma(x, fma(x, c2, c1), 1.0);
+            v = kp.sigma*(f*e);
             break;
         }
         default:
-        {
-            const double ri = rsqrt(dist);
-            v = dist == 0 ? kp.diag : cos(kp.wave_k*(dist*ri))*ri;
-            break;
-        }
+            return kval_of_dist<FAM>(kp, dist);
     }
-    return v;
+    return dist == 0 ? kp.sigma+kp.noise : v;
 }
 
 template <int NDIM, int FAM>

@@ -2,26 +2,28 @@
 // with exact integer accumulation inside each K chunk of 64 columns.
 //
 // For every row i and chunk c of 64 columns the entries are scaled by a
-// power of two 2^e >= max_{j in c} |A_ij| and rounded to an integer
-// X = rint(A_ij 2^(27-e)), |X| <= 2^27, which is split into four balanced
-// base-128 digits (Ozaki splitting):
-//   X = d1 2^21 + d2 2^14 + d3 2^7 + d4,   |d| <= 64,
-// so A_ij = 2^(e-27) X up to 2^-28 of the chunk's largest entry.  The test
-// matrix is integer: Omega_ij = round(32 x_ij) clamped to [-127, 127], x_ij
-// the dlarnv normal stream of the reference.  Each digit plane's product
-// with Omega over one chunk is exact in int32 (|sum| <= 64 * 64 * 127 <
-// 2^19); the planes are recombined exactly as 2^14 (2^7 s1 + s2) +
-// (2^7 s3 + s4) and added to the FP64 row of Y with the chunk's scale.
-// Entries are generated in FP64 from the points, as in the plain form; only
-// the products run in int8.
+// power of two 2^e > max_{j in c} |A_ij| and rounded to an integer
+// X = rint(A_ij 2^(30-e)), |X| <= 2^30, so A_ij = 2^(e-30) X up to 2^-31 of
+// the chunk's largest entry.  X is split into four signed byte digits,
+//   X = d3 2^24 + d2 2^16 + d1 2^8 + d0,  d0..d2 in [-128, 127], |d3| <= 64,
+// read off the bytes of Z = (X + 0x808080) ^ 0x808080: adding 128 to each
+// low byte moves its range to [0, 255] with the carry into the next byte,
+// and the exclusive or turns c in [0, 255] into the int8 pattern of c - 128.
+// The test matrix is integer: Omega_ij = round(32 x_ij) clamped to
+// [-127, 127], x_ij the dlarnv normal stream of the reference.  Each digit
+// plane's product with Omega over one chunk is exact in int32
+// (|sum| <= 64 * 128 * 127 < 2^21); the planes are recombined exactly as
+// 2^16 (2^8 s3 + s2) + (2^8 s1 + s0) and added to the FP64 row of Y with
+// the chunk's scale.  Entries are generated in FP64 from the points (with
+// the table-driven exp for the exponential families); only the products
+// run in int8.
 #include "pipeline.h"
 
 typedef int int4v __attribute__((ext_vector_type(4)));
 
 namespace {
 
-constexpr int RB = 64;           // rows per block
-constexpr int CB = 64;           // sketch columns per block
+constexpr int RB = 64;           // rows per block (16 per wave)
 constexpr int KB = 64;           // K chunk (one MFMA depth)
 constexpr int APAD = 80;         // bytes per LDS row (64 + 16 padding)
 
@@ -60,32 +62,57 @@ __global__ void omega_i8_kernel(signed char *out, int ncols, int ldo,
     out[p] = (signed char)v;
 }
 
-__device__ inline int pack4(int a, int b, int c, int d)
+// Byte transpose of four words: p[s] holds byte s of z0, z1, z2, z3 (in
+// that order from the low byte).  v_perm_b32 selector bytes 0-3 pick bytes
+// of the second operand, 4-7 bytes of the first.
+__device__ inline void byte_planes(unsigned z0, unsigned z1, unsigned z2,
+        unsigned z3, unsigned (&p)[4])
 {
-    return (a & 255) | ((b & 255) << 8) | ((c & 255) << 16) |
-        ((unsigned)(d & 255) << 24);
+    const unsigned t01l = __builtin_amdgcn_perm(z1, z0, 0x05010400u);
+    const unsigned t01h = __builtin_amdgcn_perm(z1, z0, 0x07030602u);
+    const unsigned t23l = __builtin_amdgcn_perm(z3, z2, 0x05010400u);
+    const unsigned t23h = __builtin_amdgcn_perm(z3, z2, 0x07030602u);
+    p[0] = __builtin_amdgcn_perm(t23l, t01l, 0x05040100u);
+    p[1] = __builtin_amdgcn_perm(t23l, t01l, 0x07060302u);
+    p[2] = __builtin_amdgcn_perm(t23h, t01h, 0x05040100u);
+    p[3] = __builtin_amdgcn_perm(t23h, t01h, 0x07060302u);
 }
 
-// One block: RB rows x CB sketch columns of one tile.  Generation: thread
-// t owns row t/4 and 16 consecutive columns (t%4)*16.. of each chunk, so a
-// row's chunk maximum is a reduction over four neighbouring lanes.  MFMA:
-// wave w owns rows 16w.. and all CB columns.
+// One block: RB rows x CB sketch columns of one tile, as four independent
+// waves of 16 rows each (no block-wide barrier after the set-up).  In a
+// wave, lane l generates row l/4 and the 16 consecutive columns
+// (l%4)*16.. of each chunk, so a row's chunk maximum is a reduction over
+// four neighbouring lanes; the column points of the chunk are staged in
+// the wave's LDS area one chunk ahead.  The wave then multiplies its own
+// 16 x 64 digit planes with the CB x 64 slice of Omega, read straight from
+// global memory (it is shared by all tiles and stays in the caches).
 template <int NDIM, int FAM, int CB>
-__global__ __launch_bounds__(256) void sketch_i8_kernel(const TileDesc *tiles,
+constexpr int sketch_waves()
+{
+    // three waves per SIMD fit without spills up to three dimensions
+    return CB == 64 && NDIM < 4 ? 3 : 2;
+}
+
+template <int NDIM, int FAM, int CB>
+__global__ __launch_bounds__(256, (sketch_waves<NDIM, FAM, CB>()))
+void sketch_i8_kernel(const TileDesc *tiles,
         double *ws, const signed char *om8)
 {
     constexpr int NF = CB/16;
-    __shared__ __align__(16) signed char As[4][RB*APAD];
-    __shared__ __align__(16) signed char Bs[CB*APAD];
-    __shared__ double csc[RB];
+    __shared__ __align__(16) signed char As[4][4][16*APAD];
+    __shared__ __align__(16) double xs[4][2][NDIM][KB];
+    __shared__ double csc[4][16];
+    __shared__ double tab[32];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    exp2_table(tab, tid);
+    __syncthreads();
     const TileDesc t = tiles[blockIdx.y];
     const int nbm = (t.m+RB-1)/RB;
     const int bm = blockIdx.x % nbm, bn = blockIdx.x / nbm;
-    if(t.m <= 0 || bn*CB >= t.k)
+    const int row0 = bm*RB+wave*16, col0 = bn*CB;
+    if(t.m <= 0 || t.n <= 0 || bn*CB >= t.k || row0 >= t.m)
         return;
-    const int row0 = bm*RB, col0 = bn*CB;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int gi = tid >> 2, gk = (tid & 3)*16;
+    const int gi = lane >> 2, gk = (lane & 3)*16;
     const bool row_ok = row0+gi < t.m;
     double xi[NDIM];
 #pragma unroll
@@ -93,115 +120,123 @@ __global__ __launch_bounds__(256) void sketch_i8_kernel(const TileDesc *tiles,
         xi[d] = row_ok ? t.xr[d*t.nr+t.r0+row0+gi] : 0.0;
     const signed char *om = om8+t.om8_off;
     const int ldo = (t.n+63)/64*64;
+    signed char *Aw[4] = {As[wave][0], As[wave][1], As[wave][2],
+        As[wave][3]};
+    // column points of chunk 0: lane l holds column l of the chunk
+    const int64_t cbase = t.c0;
+#pragma unroll
+    for(int d = 0; d < NDIM; d++)
+        xs[wave][0][d][lane] = t.xc[d*t.nc+cbase+(lane < t.n ? lane :
+                t.n-1)];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     double yacc[NF][4];
 #pragma unroll
     for(int c = 0; c < NF; c++)
 #pragma unroll
         for(int v = 0; v < 4; v++)
             yacc[c][v] = 0.0;
-    for(int k0 = 0; k0 < t.n; k0 += KB)
+    int buf = 0;
+    for(int k0 = 0; k0 < t.n; k0 += KB, buf ^= 1)
     {
-        // ---- A: FP64 entries of the chunk, scale, four int8 digit planes
-        // column points of this thread's 16 entries, loaded together
-        double yv[16][NDIM];
+        // ---- loads for later in the iteration: this chunk's Omega
+        // fragments (column col0+16c+lane%16, K bytes kq..kq+15) and the
+        // next chunk's column points
+        const int kq = (lane >> 4)*16;
+        int4v ob[NF];
 #pragma unroll
-        for(int e = 0; e < 16; e++)
+        for(int c = 0; c < NF; c++)
         {
-            const int kk = k0+gk+e;
-            const int64_t jj = t.c0+(kk < t.n ? kk : t.n-1);
-#pragma unroll
-            for(int d = 0; d < NDIM; d++)
-                yv[e][d] = t.xc[d*t.nc+jj];
+            const int j = col0+c*16+(lane & 15);
+            ob[c] = j < t.k ? *(const int4v *)(om+(int64_t)j*ldo+k0+kq) :
+                int4v{0, 0, 0, 0};
         }
+        double xn[NDIM];
+        const int kn = k0+KB+lane;
+#pragma unroll
+        for(int d = 0; d < NDIM; d++)
+            xn[d] = t.xc[d*t.nc+cbase+(kn < t.n ? kn : t.n-1)];
+        // ---- A: FP64 entries of the chunk, scale, four byte planes
         double a[16];
         double mx = 0.0;
 #pragma unroll
         for(int e = 0; e < 16; e++)
         {
-            const int kk = k0+gk+e;
-            const double v = kval_pt<NDIM, FAM>(t.kp, xi, yv[e]);
-            a[e] = (row_ok && kk < t.n) ? v : 0.0;
-            const double f = fabs(a[e]);
-            mx = f > mx ? f : mx;
+            double dist = 0.0;
+#pragma unroll
+            for(int d = 0; d < NDIM; d++)
+            {
+                const double q = xi[d]-xs[wave][buf][d][gk+e];
+                dist = fma(q, q, dist);
+            }
+            // no masking: a row past the tile only feeds its own output
+            // row, which is not stored, and a column past the tile repeats
+            // the last column's point (so it leaves the chunk maximum
+            // unchanged) and meets a zero row of Omega
+            a[e] = kval_of_dist_tab<FAM>(t.kp, dist, tab);
+            mx = fmax(mx, fabs(a[e]));
         }
-        double o = __shfl_xor(mx, 1, 64);
-        mx = o > mx ? o : mx;
-        o = __shfl_xor(mx, 2, 64);
-        mx = o > mx ? o : mx;
+        mx = fmax(mx, __shfl_xor(mx, 1, 64));
+        mx = fmax(mx, __shfl_xor(mx, 2, 64));
         int ex = 0;
         if(mx > 0.0)
             frexp(mx, &ex);     // mx = f 2^ex, f in [0.5, 1)
-        const double mul = mx > 0.0 ? ldexp(1.0, 27-ex) : 0.0;
-        if((tid & 3) == 0)
-            csc[gi] = mx > 0.0 ? ldexp(1.0, ex-27) : 0.0;
-        int dg[4][16];
+        if((lane & 3) == 0)
+            csc[wave][gi] = mx > 0.0 ? ldexp(1.0, ex-30) : 0.0;
 #pragma unroll
-        for(int e = 0; e < 16; e++)
+        for(int g = 0; g < 4; g++)
         {
-            int x = (int)rint(a[e]*mul);
-            const int d4 = ((x+64) & 127)-64;
-            x = (x-d4) >> 7;
-            const int d3 = ((x+64) & 127)-64;
-            x = (x-d3) >> 7;
-            const int d2 = ((x+64) & 127)-64;
-            dg[3][e] = d4;
-            dg[2][e] = d3;
-            dg[1][e] = d2;
-            dg[0][e] = (x-d2) >> 7;
+            unsigned z[4];
+#pragma unroll
+            for(int e = 0; e < 4; e++)
+            {
+                const int x = (int)rint(ldexp(a[4*g+e], 30-ex));
+                z[e] = ((unsigned)x+0x808080u) ^ 0x808080u;
+            }
+            unsigned pl[4];
+            byte_planes(z[0], z[1], z[2], z[3], pl);
+#pragma unroll
+            for(int s = 0; s < 4; s++)
+                *(unsigned *)&Aw[s][gi*APAD+gk+4*g] = pl[s];
         }
 #pragma unroll
-        for(int s = 0; s < 4; s++)
-        {
-            int4v v;
-            v.x = pack4(dg[s][0], dg[s][1], dg[s][2], dg[s][3]);
-            v.y = pack4(dg[s][4], dg[s][5], dg[s][6], dg[s][7]);
-            v.z = pack4(dg[s][8], dg[s][9], dg[s][10], dg[s][11]);
-            v.w = pack4(dg[s][12], dg[s][13], dg[s][14], dg[s][15]);
-            *(int4v *)&As[s][gi*APAD+gk] = v;
-        }
-        // ---- B: Omega columns col0.., K rows k0.. (column-major, leading
-        // dimension ldo = n rounded up to 64, zero padded)
-#pragma unroll
-        for(int h = 0; h < CB/64; h++)
-        {
-            const int j = (tid >> 2)+64*h, kq = (tid & 3)*16;
-            int4v v = int4v{0, 0, 0, 0};
-            if(col0+j < t.k)
-                v = *(const int4v *)(om+(int64_t)(col0+j)*ldo+k0+kq);
-            *(int4v *)&Bs[j*APAD+kq] = v;
-        }
-        __syncthreads();
+        for(int d = 0; d < NDIM; d++)
+            xs[wave][buf^1][d][lane] = xn[d];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // ---- MFMA over the chunk, then exact recombination into FP64.
         // The i32 output of v_mfma_i32_16x16x64_i8 holds rows
         // 4*(lane/16)+v, column lane%16.
-        const int ar = wave*16+(lane & 15), kq = (lane >> 4)*16;
         int4v as[4];
 #pragma unroll
         for(int s = 0; s < 4; s++)
-            as[s] = *(const int4v *)&As[s][ar*APAD+kq];
+            as[s] = *(const int4v *)&Aw[s][(lane & 15)*APAD+kq];
         double sc[4];
 #pragma unroll
         for(int v = 0; v < 4; v++)
-            sc[v] = csc[wave*16+4*(lane >> 4)+v];
+            sc[v] = csc[wave][4*(lane >> 4)+v];
 #pragma unroll
         for(int c = 0; c < NF; c++)
         {
-            const int4v b = *(const int4v *)&Bs[(c*16+(lane & 15))*APAD+kq];
             int4v acc[4];
 #pragma unroll
             for(int s = 0; s < 4; s++)
-                acc[s] = __builtin_amdgcn_mfma_i32_16x16x64_i8(as[s], b,
+                acc[s] = __builtin_amdgcn_mfma_i32_16x16x64_i8(as[s], ob[c],
                         int4v{0, 0, 0, 0}, 0, 0, 0);
 #pragma unroll
             for(int v = 0; v < 4; v++)
             {
-                const int hi = acc[0][v]*128+acc[1][v];
-                const int lo = acc[2][v]*128+acc[3][v];
-                yacc[c][v] = fma((double)hi*16384.0+(double)lo, sc[v],
+                const int hi = acc[3][v]*256+acc[2][v];
+                const int lo = acc[1][v]*256+acc[0][v];
+                yacc[c][v] = fma(fma((double)hi, 65536.0, (double)lo), sc[v],
                         yacc[c][v]);
             }
         }
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     // ---- epilogue: Y = sum over chunks / 32 (Omega's integer scale)
     double *Y = ws+t.q_off;
@@ -214,7 +249,7 @@ __global__ __launch_bounds__(256) void sketch_i8_kernel(const TileDesc *tiles,
 #pragma unroll
         for(int v = 0; v < 4; v++)
         {
-            const int gr = row0+wave*16+4*(lane >> 4)+v;
+            const int gr = row0+4*(lane >> 4)+v;
             if(gr < t.m)
                 Y[(int64_t)gj*t.m+gr] = yacc[c][v]*(1.0/32.0);
         }
