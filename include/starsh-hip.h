@@ -102,6 +102,9 @@ int starsh_hip_get_stats(starsh_hip_stats *stats);
  *                         size, default), 1 or 2;
  *   "qr_panel"            panel width of the block Gram-Schmidt, 4
  *                         (default) or 8 (tiles of at most 2048 rows);
+ *   "proj_int8"           projection B^T = A^T Q of the mixed form on
+ *                         int8 digit planes of both operands (1, default)
+ *                         or on FP64 MFMA (0);
  *   "release_workspace"   (value ignored) frees the device workspace and
  *                         the pinned staging buffer, which are otherwise
  *                         kept and reused between calls.

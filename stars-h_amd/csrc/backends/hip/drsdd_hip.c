@@ -53,6 +53,8 @@ int starsh_hip_set_option(const char *name, double value)
         return starsh_hip_gen_gemm_rw((int)value);
     if(!strcmp(name, "qr_panel"))
         return starsh_hip_qr_panel((int)value);
+    if(!strcmp(name, "proj_int8"))
+        return starsh_hip_proj_int8((int)value);
     if(!strcmp(name, "release_workspace"))
     {
         rsdd_release_pools();
@@ -405,8 +407,10 @@ int hip_driver_finish(STARSH_blrm **matrix, STARSH_blrf *format,
 static size_t tile_bytes(int m, int n, int k, int maxrank)
 {
     size_t r = (size_t)(maxrank < k ? maxrank : k);
+    /* FP64 buffers, factors, int8 Omega, int8 digit planes of Q */
     return sizeof(double)*((size_t)m*k+2*(size_t)n*k+4*(size_t)k*k+
-            (size_t)k+(size_t)(m+n)*r)+(size_t)(n+63)*k;
+            (size_t)k+(size_t)(m+n)*r)+(size_t)(n+63)*k+
+        4*(size_t)(m+63)*(size_t)(k+15)+8*(size_t)(k+15);
 }
 
 int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,

@@ -20,50 +20,6 @@ constexpr int BK = 16;
 int gen_gemm_bn = 0;
 int gen_gemm_rw = 0;
 
-// Workgroups are dispatched to the 8 XCDs of the chip round robin (group
-// L runs on XCD L % 8), and each XCD has its own L2.  The 1-D grid is
-// decoded so that an XCD sees few distinct right operands:
-//  - shared B (the sketch: every tile multiplies the same Omega): XCD x
-//    takes only column block x % nbn (when nbn divides 8), so it keeps one
-//    slice of Omega in its L2;
-//  - per-tile B (the projection: B = the tile's own Q): all blocks of a
-//    tile run on one XCD, which reads that Q once.
-constexpr int NXCD = 8;
-
-__device__ inline bool map_block(int64_t L, int ntiles, int nbm, int nbn,
-        bool shared_b, int &tile, int &bm, int &bn)
-{
-    const int xcd = (int)(L % NXCD);
-    const int64_t slot = L / NXCD;
-    if(shared_b && NXCD % nbn == 0)
-    {
-        const int per = NXCD/nbn;
-        bn = xcd % nbn;
-        const int64_t f = slot*per+xcd/nbn;
-        tile = (int)(f/nbm);
-        bm = (int)(f % nbm);
-    }
-    else
-    {
-        const int64_t X = (int64_t)nbm*nbn;
-        tile = (int)((slot/X)*NXCD+xcd);
-        const int b = (int)(slot % X);
-        bm = b % nbm;
-        bn = b / nbm;
-    }
-    return tile < ntiles;
-}
-
-inline int64_t mapped_grid(int ntiles, int nbm, int nbn, bool shared_b)
-{
-    if(shared_b && NXCD % nbn == 0)
-    {
-        const int per = NXCD/nbn;
-        return (int64_t)NXCD*(((int64_t)ntiles*nbm+per-1)/per);
-    }
-    return (int64_t)NXCD*((ntiles+NXCD-1)/NXCD)*nbm*nbn;
-}
-
 // PROJ=false: rows are row points, K runs over column points.
 // PROJ=true : rows are column points, K runs over row points (A^T).
 // gnbm x gnbn is the block grid of the largest tile of the launch.

@@ -72,6 +72,8 @@ struct TileDesc
     int64_t u_off;     // U (m x maxrank, ld m)
     int64_t v_off;     // V (n x maxrank, ld n)
     int64_t om8_off;   // integer test matrix (bytes, n x k, ld n up to 64)
+    int64_t qd_off;    // int8 digit planes of Q and their column scales
+                       // (bytes, see pipeline_qdigits)
     int kc;           // kept (numerically independent) sketch columns
     int p;             // size of the core problem after truncation
     int rank;          // chosen rank, -1 for a false far-field tile
@@ -298,6 +300,77 @@ struct PointSets
     const double *xc;
     int64_t nc;
 };
+
+// Byte transpose of four words: p[s] holds byte s of z0, z1, z2, z3 (in
+// that order from the low byte).  v_perm_b32 selector bytes 0-3 pick bytes
+// of the second operand, 4-7 bytes of the first.
+__device__ inline void byte_planes(unsigned z0, unsigned z1, unsigned z2,
+        unsigned z3, unsigned (&p)[4])
+{
+    const unsigned t01l = __builtin_amdgcn_perm(z1, z0, 0x05010400u);
+    const unsigned t01h = __builtin_amdgcn_perm(z1, z0, 0x07030602u);
+    const unsigned t23l = __builtin_amdgcn_perm(z3, z2, 0x05010400u);
+    const unsigned t23h = __builtin_amdgcn_perm(z3, z2, 0x07030602u);
+    p[0] = __builtin_amdgcn_perm(t23l, t01l, 0x05040100u);
+    p[1] = __builtin_amdgcn_perm(t23l, t01l, 0x07060302u);
+    p[2] = __builtin_amdgcn_perm(t23h, t01h, 0x05040100u);
+    p[3] = __builtin_amdgcn_perm(t23h, t01h, 0x07060302u);
+}
+
+// Signed byte digits of an integer |x| <= 2^30:
+//   x = d3 2^24 + d2 2^16 + d1 2^8 + d0,  d0..d2 in [-128, 127], |d3| <= 64,
+// where d_s is byte s of the result read as int8.  Adding 128 to each low
+// byte moves its digit range to [0, 255] with the carry into the next
+// byte; the exclusive or turns c in [0, 255] into the int8 pattern of
+// c - 128.
+__device__ inline unsigned byte_digits(int x)
+{
+    return ((unsigned)x+0x808080u) ^ 0x808080u;
+}
+
+// Workgroups are dispatched to the 8 XCDs of the chip round robin (group
+// L runs on XCD L % 8), and each XCD has its own L2.  The 1-D grid is
+// decoded so that an XCD sees few distinct right operands:
+//  - shared B (every tile multiplies the same Omega): XCD x takes only
+//    column block x % nbn (when nbn divides 8), so it keeps one slice of
+//    Omega in its L2;
+//  - per-tile B (the projection: B = the tile's own Q): all blocks of a
+//    tile run on one XCD, which reads that Q once.
+constexpr int NXCD = 8;
+
+__device__ inline bool map_block(int64_t L, int ntiles, int nbm, int nbn,
+        bool shared_b, int &tile, int &bm, int &bn)
+{
+    const int xcd = (int)(L % NXCD);
+    const int64_t slot = L / NXCD;
+    if(shared_b && NXCD % nbn == 0)
+    {
+        const int per = NXCD/nbn;
+        bn = xcd % nbn;
+        const int64_t f = slot*per+xcd/nbn;
+        tile = (int)(f/nbm);
+        bm = (int)(f % nbm);
+    }
+    else
+    {
+        const int64_t X = (int64_t)nbm*nbn;
+        tile = (int)((slot/X)*NXCD+xcd);
+        const int b = (int)(slot % X);
+        bm = b % nbm;
+        bn = b / nbm;
+    }
+    return tile < ntiles;
+}
+
+inline int64_t mapped_grid(int ntiles, int nbm, int nbn, bool shared_b)
+{
+    if(shared_b && NXCD % nbn == 0)
+    {
+        const int per = NXCD/nbn;
+        return (int64_t)NXCD*(((int64_t)ntiles*nbm+per-1)/per);
+    }
+    return (int64_t)NXCD*((ntiles+NXCD-1)/NXCD)*nbm*nbn;
+}
 
 // Library stream (created by starsh_hip_init).
 extern "C" hipStream_t starsh_hip_stream(void);

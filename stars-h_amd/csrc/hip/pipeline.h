@@ -68,6 +68,18 @@ int pipeline_sketch_i8(const TileDesc *d_tiles, int ntiles, int mmax,
         int kmax, const GemmArgs &g, const signed char *d_om8,
         hipStream_t s);
 
+// Bytes of one tile's int8 digit planes of Q and their column scales.
+int64_t pipeline_qdigits_bytes(int m, int k);
+
+// Digit planes of the kc kept columns of Q, per tile at qd_off of d_qd.
+int pipeline_qdigits(const TileDesc *d_tiles, int ntiles, int kmax,
+        const double *ws, signed char *d_qd, hipStream_t s);
+
+// Bt = A^T Q with the tile generated in registers and both operands in
+// int8 digit planes (mixed form).
+int pipeline_proj_i8(const TileDesc *d_tiles, int ntiles, int nmax, int kmax,
+        const GemmArgs &g, const signed char *d_qd, hipStream_t s);
+
 // One-sided Jacobi SVD of R2, singular values, rank by the reference rule
 // (each tile's own maxrank and tol) and X_r.
 int pipeline_core(TileDesc *d_tiles, int ntiles, double *ws, int pmax,

@@ -117,6 +117,10 @@ ActiveProbe probe;
 // measured faster on MI355X: smaller blocks, more of them per CU.
 int qr_panel_width = 4;
 
+// Projection of the mixed form on int8 digit planes (1, default) or FP64
+// MFMA (0); set by starsh_hip_set_option("proj_int8").
+int proj_int8 = 1;
+
 int bcgs2(TileDesc *d_tiles, int ntiles, int which, double *ws,
         const double *d_ref, int kmax, int width_max, int rows_max,
         int *d_keep, double tau_first, int *d_active, hipStream_t s)
@@ -324,6 +328,15 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
         else
             h[t].om_off = omegas[key];
     }
+    // int8 digit planes of Q for the projection of the mixed form
+    const bool proj8 = in->sketch_int8 && proj_int8;
+    int64_t offq = 0;
+    if(proj8)
+        for(int t = 0; t < nt; t++)
+        {
+            h[t].qd_off = offq;
+            offq += pipeline_qdigits_bytes(h[t].m, h[t].k);
+        }
     int64_t uvoff = 0;
     for(int t = 0; t < nt; t++)
     {
@@ -344,10 +357,12 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     if(in->rows_out)
         for(int t = 0; t < nt; t++)
             in->rows_out[t] = h[t].k;
-    DeviceBuf tiles, keep, norms, om8, active;
+    DeviceBuf tiles, keep, norms, om8, active, qd;
     TRY(ws_pool.reserve(sizeof(double)*off));
     if(in->sketch_int8)
         TRY(om8.alloc(off8));
+    if(proj8)
+        TRY(qd.alloc(offq));
     TRY(tiles.alloc(sizeof(TileDesc)*nt));
     TRY(keep.alloc(sizeof(int)*(size_t)nt*kmax));
     TRY(norms.alloc(sizeof(double)*(size_t)nt*kmax));
@@ -411,6 +426,13 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     // 3. projection Bt = A^T Q
     if(in->d_a)
         TRY(pipeline_gemm(GM_PROJ_MEM, d_tiles, nt, nmax, kmax, g, s));
+    else if(proj8)
+    {
+        TRY(pipeline_qdigits(d_tiles, nt, kmax, d_ws, (signed char *)qd.p,
+                    s));
+        TRY(pipeline_proj_i8(d_tiles, nt, nmax, kmax, g,
+                    (const signed char *)qd.p, s));
+    }
     else
         TRY(pipeline_gen_gemm(1, d_tiles, nt, nmax, kmax, g, s));
     dump("Bt", d_ws+h[0].bt_off, (int64_t)h[0].n*h[0].k, s);
@@ -545,6 +567,14 @@ void rsdd_release_pools(void)
     uv_pool.release();
     pack_pool.release();
     stage_pool.release();
+}
+
+extern "C" int starsh_hip_proj_int8(int on)
+{
+    if(on != 0 && on != 1)
+        return STARSH_WRONG_PARAMETER;
+    proj_int8 = on;
+    return STARSH_SUCCESS;
 }
 
 extern "C" int starsh_hip_qr_panel(int width)

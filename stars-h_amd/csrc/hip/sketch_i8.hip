@@ -62,22 +62,6 @@ __global__ void omega_i8_kernel(signed char *out, int ncols, int ldo,
     out[p] = (signed char)v;
 }
 
-// Byte transpose of four words: p[s] holds byte s of z0, z1, z2, z3 (in
-// that order from the low byte).  v_perm_b32 selector bytes 0-3 pick bytes
-// of the second operand, 4-7 bytes of the first.
-__device__ inline void byte_planes(unsigned z0, unsigned z1, unsigned z2,
-        unsigned z3, unsigned (&p)[4])
-{
-    const unsigned t01l = __builtin_amdgcn_perm(z1, z0, 0x05010400u);
-    const unsigned t01h = __builtin_amdgcn_perm(z1, z0, 0x07030602u);
-    const unsigned t23l = __builtin_amdgcn_perm(z3, z2, 0x05010400u);
-    const unsigned t23h = __builtin_amdgcn_perm(z3, z2, 0x07030602u);
-    p[0] = __builtin_amdgcn_perm(t23l, t01l, 0x05040100u);
-    p[1] = __builtin_amdgcn_perm(t23l, t01l, 0x07060302u);
-    p[2] = __builtin_amdgcn_perm(t23h, t01h, 0x05040100u);
-    p[3] = __builtin_amdgcn_perm(t23h, t01h, 0x07060302u);
-}
-
 // One block: RB rows x CB sketch columns of one tile, as four independent
 // waves of 16 rows each (no block-wide barrier after the set-up).  In a
 // wave, lane l generates row l/4 and the 16 consecutive columns
@@ -191,8 +175,7 @@ void sketch_i8_kernel(const TileDesc *tiles,
 #pragma unroll
             for(int e = 0; e < 4; e++)
             {
-                const int x = (int)rint(ldexp(a[4*g+e], 30-ex));
-                z[e] = ((unsigned)x+0x808080u) ^ 0x808080u;
+                z[e] = byte_digits((int)rint(ldexp(a[4*g+e], 30-ex)));
             }
             unsigned pl[4];
             byte_planes(z[0], z[1], z[2], z[3], pl);

@@ -81,6 +81,8 @@ int starsh_hip_gen_gemm_bn(int bn);
 int starsh_hip_gen_gemm_rw(int rw);
 /* Panel width of the block Gram-Schmidt (8 or 4). */
 int starsh_hip_qr_panel(int width);
+/* Projection of the mixed form on int8 digit planes (1) or FP64 MFMA (0). */
+int starsh_hip_proj_int8(int on);
 size_t hip_driver_workspace_budget(void);
 double hip_driver_now(void);
 

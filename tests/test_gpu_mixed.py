@@ -57,6 +57,50 @@ def test_int8_sketch_matches_exact_product(tmp_path):
     p.free_matrix()
 
 
+def test_int8_projection_matches_exact_product(tmp_path):
+    """Bt from the int8 digit planes of A^T and Q equals A^T Q up to the
+    rounding of both operands: A per (column, chunk of 64 rows) and Q per
+    column to multiples of 2^-30 of a power of two at most twice the
+    largest entry, so each term is off by at most 2^-29 of the product of
+    those maxima."""
+    _say("case: int8 projection of 2D sqexp N=3200 nb=800")
+    os.environ["STARSH_HIP_DEBUG_DUMP"] = str(tmp_path)
+    L = S.lib()
+    assert L.starsh_hip_set_option(b"mixed_first_bucket", C.c_double(4)) == 0
+    try:
+        fn = S.init("HIP_MIXED", oversample=16)
+        p = S.TLRProblem("spatial", 3200, 2, 800, symm="S", seed=4,
+                         kernel="sqrexp", beta=0.1)
+        assert p.approximate(200, 1e-7, 0, fn) == 0
+    finally:
+        del os.environ["STARSH_HIP_DEBUG_DUMP"]
+        L.starsh_hip_set_option(b"mixed_first_bucket", C.c_double(-1))
+    m = n = 800
+    Q = np.fromfile(tmp_path / "Q.bin")
+    k = Q.size // m
+    Q = Q.reshape(k, m).T
+    Bt = np.fromfile(tmp_path / "Bt.bin").reshape(k, n).T
+    pts = p.points()
+    st, sz = p.cluster_ranges()
+    A = O.kernel_block(O.kernel("sqrexp", 2, beta=0.1), pts, pts,
+                       np.arange(st[0], st[0] + sz[0]),
+                       np.arange(st[0], st[0] + sz[0]))
+    # the first columns of Q are kept ones (the tile's rank is above 4)
+    c = 4
+    q = Q[:, :c]
+    np.testing.assert_allclose(q.T @ q, np.eye(c), atol=1e-12)
+    want = A.T @ q
+    amax = np.array([np.abs(A[i:i + 64]).max(axis=0)
+                     for i in range(0, m, 64)])               # chunk x j
+    qmax = np.abs(q).max(axis=0)
+    # sum over chunks of 64 * 2^-29 * amax * qmax, per (j, c)
+    bound = 2.0 ** -29 * 64 * np.outer(amax.sum(axis=0), qmax)
+    err = np.abs(Bt[:, :c] - want)
+    assert np.all(err <= bound)
+    assert err.max() > 0       # the int8 path ran (FP64 would be ~1e-16)
+    p.free_matrix()
+
+
 def _mixed_vs_plain(kind, N, ndim, nb, maxrank, os_, tol, **kw):
     res = {}
     for backend in ("HIP", "HIP_MIXED"):
