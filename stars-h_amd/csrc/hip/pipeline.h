@@ -33,16 +33,18 @@ int pipeline_gen_gemm(int proj, const TileDesc *d_tiles, int ntiles,
 int pipeline_colnorm(const TileDesc *d_tiles, int ntiles, int which,
         double *ws, double *d_norm, int kmax, hipStream_t s);
 
-// One block Gram-Schmidt pass for a panel of jb columns at j0 (projection
-// against the previous columns, then CGS2 inside the panel with
-// deflation).  which: 0 = Q (width k), 1 = Q2 (width p).  Columns whose
-// norm after projection is at most tau * ref (ref = d_ref[t*kmax+j], or 1
-// when d_ref is null) are set to zero and flagged in d_keep.  The first
-// pass (d_ref not null) marks a tile whose whole panel was dropped as done;
-// *d_active counts the tiles that still have panels to process.
+// Block Gram-Schmidt applied twice for a panel of jb columns at j0 (each
+// pass: projection against the previous columns, then CGS2 inside the
+// panel with deflation).  which: 0 = Q (width k), 1 = Q2 (width p).
+// Columns whose norm after projection is at most tau_first * ref (ref =
+// d_ref[t*kmax+j], or 1 when d_ref is null) in the first pass, or
+// tau_second in the second, are set to zero and flagged in d_keep.  A tile
+// whose whole panel was dropped in the first pass is marked done; *d_active
+// counts the tiles that still have panels to process.
 int pipeline_panel(const TileDesc *d_tiles, int ntiles, int which,
         double *ws, int j0, int jb, const double *d_ref, int kmax,
-        double tau, int *d_keep, int rows_max, int *d_active, hipStream_t s);
+        double tau_first, double tau_second, int *d_keep, int rows_max,
+        int *d_active, hipStream_t s);
 
 // *d_active = number of tiles with a nonzero width (k or p).
 int pipeline_count_active(const TileDesc *d_tiles, int ntiles, int which,

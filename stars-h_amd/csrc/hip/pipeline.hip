@@ -138,11 +138,8 @@ int bcgs2(TileDesc *d_tiles, int ntiles, int which, double *ws,
             if(probe.h[slot] == 0)
                 break;
         }
-        for(int pass = 0; pass < 2; pass++)
-            TRY(pipeline_panel(d_tiles, ntiles, which, ws, j0, pb,
-                        pass == 0 ? d_ref : nullptr, kmax,
-                        pass == 0 ? tau_first : TAU_SECOND, d_keep, rows_max,
-                        d_active, s));
+        TRY(pipeline_panel(d_tiles, ntiles, which, ws, j0, pb, d_ref, kmax,
+                    tau_first, TAU_SECOND, d_keep, rows_max, d_active, s));
         const int slot = panel % NSLOT;
         HIP_TRY(hipMemcpyAsync(probe.h+slot, d_active, sizeof(int),
                     hipMemcpyDeviceToHost, s));

@@ -101,24 +101,25 @@ __device__ inline int halving_index(int lane)
     return idx;
 }
 
-// One pass of block Gram-Schmidt for the panel W = columns j0..j0+PB-1 of
-// Q (which = 0, m rows, width k) or Q2 (which = 1, n rows, width p), fused
-// in one workgroup per tile:
+// Block Gram-Schmidt applied twice (BCGS2) to the panel W = columns
+// j0..j0+PB-1 of Q (which = 0, m rows, width k) or Q2 (which = 1, n rows,
+// width p), fused in one workgroup per tile.  Each of the two passes:
 //   1. projection against the previous columns, CH at a time (block
 //      modified Gram-Schmidt): C = Q_chᵀ W, W -= Q_ch C; each chunk of Q is
 //      read once, and its CH x PB dot products are reduced by
 //      halving_sum, then across the waves in LDS;
 //   2. CGS2 inside the panel with deflation: a column whose norm after
-//      projection is at most tau times its original norm (ref, or 1 when
-//      ref is null) is set to zero and flagged in keep.
+//      projection is at most tau times its reference norm (ref in the
+//      first pass, or 1 when ref is null; 1 in the second pass, the column
+//      having been normalised) is set to zero and flagged in keep.
 // The panel lives in registers: thread t holds rows t, t+NT, ... (RPT of
-// them).  Called twice per panel (BCGS2); the first call (ref != null)
-// marks a tile whose whole panel was dropped as done, and *active counts
-// the tiles that still have panels to process.
+// them).  A tile whose whole panel is dropped in the first pass is marked
+// done (its range is exhausted); *active counts the tiles that still have
+// panels to process.
 template <int NT, int RPT, int PB>
 __global__ __launch_bounds__(NT) void bcgs_kernel(TileDesc *tiles, int which,
-        double *ws, int j0, const double *ref, int kmax, double tau2,
-        int *keep, int first_pass, int *active)
+        double *ws, int j0, const double *ref, int kmax, double tau2_first,
+        double tau2_second, int *keep, int *active)
 {
     constexpr int NW = NT/64, CH = 4, V = CH*PB;
     __shared__ double red[NW][V];
@@ -152,125 +153,161 @@ __global__ __launch_bounds__(NT) void bcgs_kernel(TileDesc *tiles, int which,
         for(int c = 0; c < PB; c++)
             w[r][c] = (i < rows && c < jb) ? W[(int64_t)c*rows+i] : 0.0;
     }
-    // ---- 1. projection against columns 0..j0-1 (j0 is a multiple of CH)
     const int my_idx = halving_index<V>(lane);
     const bool writer = (lane & ((64/V)-1)) == 0;
-    for(int l0 = 0; l0 < j0; l0 += CH)
+    int rb = 0;   // partial buffer of the next in-panel reduction
+    for(int pass = 0; pass < 2; pass++)
     {
-        double q[RPT][CH];
-#pragma unroll
-        for(int r = 0; r < RPT; r++)
+        // ---- 1. projection against columns 0..j0-1 (j0 is a multiple of
+        // CH)
+        for(int l0 = 0; l0 < j0; l0 += CH)
         {
-            const int i = threadIdx.x+r*NT;
+            double q[RPT][CH];
+#pragma unroll
+            for(int r = 0; r < RPT; r++)
+            {
+                const int i = threadIdx.x+r*NT;
+#pragma unroll
+                for(int c = 0; c < CH; c++)
+                    q[r][c] = i < rows ? Qb[(int64_t)(l0+c)*rows+i] : 0.0;
+            }
+            double v[V];
 #pragma unroll
             for(int c = 0; c < CH; c++)
-                q[r][c] = i < rows ? Qb[(int64_t)(l0+c)*rows+i] : 0.0;
-        }
-        double v[V];
 #pragma unroll
-        for(int c = 0; c < CH; c++)
+                for(int b = 0; b < PB; b++)
+                {
+                    double d = 0.0;
 #pragma unroll
-            for(int b = 0; b < PB; b++)
+                    for(int r = 0; r < RPT; r++)
+                        d += q[r][c]*w[r][b];
+                    v[c*PB+b] = d;
+                }
+            const double tot = halving_sum<V>(v, lane);
+            if(writer)
+                red[wave][my_idx] = tot;
+            __syncthreads();
+            if(threadIdx.x < V)
             {
+                double sum = 0.0;
+#pragma unroll
+                for(int q2w = 0; q2w < NW; q2w++)
+                    sum += red[q2w][threadIdx.x];
+                fin[threadIdx.x] = sum;
+            }
+            __syncthreads();
+#pragma unroll
+            for(int c = 0; c < CH; c++)
+#pragma unroll
+                for(int b = 0; b < PB; b++)
+                {
+                    const double coef = fin[c*PB+b];
+#pragma unroll
+                    for(int r = 0; r < RPT; r++)
+                        w[r][b] -= q[r][c]*coef;
+                }
+        }
+        // ---- 2. CGS2 inside the panel with deflation
+        int ndropped = 0;
+#pragma unroll
+        for(int c = 0; c < PB; c++)
+        {
+            if(c < jb)
+            {
+#pragma unroll
+                for(int sub = 0; sub < 2; sub++)
+                {
+                    if(c > 0)
+                    {
+#pragma unroll
+                        for(int l = 0; l < PB; l++)
+                            if(l < c)
+                            {
+                                double d = 0.0;
+#pragma unroll
+                                for(int r = 0; r < RPT; r++)
+                                    d += w[r][l]*w[r][c];
+                                d = wave_sum(d);
+                                if(lane == 0)
+                                    pred[rb][wave][l] = d;
+                            }
+                        __syncthreads();
+#pragma unroll
+                        for(int l = 0; l < PB; l++)
+                            if(l < c)
+                            {
+                                double coef = 0.0;
+#pragma unroll
+                                for(int q2w = 0; q2w < NW; q2w++)
+                                    coef += pred[rb][q2w][l];
+#pragma unroll
+                                for(int r = 0; r < RPT; r++)
+                                    w[r][c] -= coef*w[r][l];
+                            }
+                        rb ^= 1;
+                    }
+                }
                 double d = 0.0;
 #pragma unroll
                 for(int r = 0; r < RPT; r++)
-                    d += q[r][c]*w[r][b];
-                v[c*PB+b] = d;
-            }
-        const double tot = halving_sum<V>(v, lane);
-        if(writer)
-            red[wave][my_idx] = tot;
-        __syncthreads();
-        if(threadIdx.x < V)
-        {
-            double sum = 0.0;
+                    d += w[r][c]*w[r][c];
+                d = wave_sum(d);
+                if(lane == 0)
+                    pred[rb][wave][0] = d;
+                __syncthreads();
+                double n2 = 0.0;
 #pragma unroll
-            for(int q2w = 0; q2w < NW; q2w++)
-                sum += red[q2w][threadIdx.x];
-            fin[threadIdx.x] = sum;
-        }
-        __syncthreads();
-#pragma unroll
-        for(int c = 0; c < CH; c++)
-#pragma unroll
-            for(int b = 0; b < PB; b++)
-            {
-                const double coef = fin[c*PB+b];
+                for(int q2w = 0; q2w < NW; q2w++)
+                    n2 += pred[rb][q2w][0];
+                rb ^= 1;
+                const double r2 = pass == 0 && ref ?
+                    ref[(int64_t)blockIdx.x*kmax+j0+c] : 1.0;
+                const double tau2 = pass == 0 ? tau2_first : tau2_second;
+                const bool drop = !(n2 > tau2*r2) || n2 == 0.0;
+                const double inv = drop ? 0.0 : 1.0/sqrt(n2);
 #pragma unroll
                 for(int r = 0; r < RPT; r++)
-                    w[r][b] -= q[r][c]*coef;
-            }
-    }
-    // ---- 2. CGS2 inside the panel with deflation
-    int ndropped = 0;
-    int rb = 0;   // partial buffer of the next reduction
-#pragma unroll
-    for(int c = 0; c < PB; c++)
-    {
-        if(c < jb)
-        {
-#pragma unroll
-            for(int pass = 0; pass < 2; pass++)
-            {
-                if(c > 0)
+                    w[r][c] *= inv;
+                if(drop)
+                    ndropped++;
+                if(threadIdx.x == 0)
                 {
-#pragma unroll
-                    for(int l = 0; l < PB; l++)
-                        if(l < c)
-                        {
-                            double d = 0.0;
-#pragma unroll
-                            for(int r = 0; r < RPT; r++)
-                                d += w[r][l]*w[r][c];
-                            d = wave_sum(d);
-                            if(lane == 0)
-                                pred[rb][wave][l] = d;
-                        }
-                    __syncthreads();
-#pragma unroll
-                    for(int l = 0; l < PB; l++)
-                        if(l < c)
-                        {
-                            double coef = 0.0;
-#pragma unroll
-                            for(int q2w = 0; q2w < NW; q2w++)
-                                coef += pred[rb][q2w][l];
-#pragma unroll
-                            for(int r = 0; r < RPT; r++)
-                                w[r][c] -= coef*w[r][l];
-                        }
-                    rb ^= 1;
+                    if(pass == 0)
+                        kp[c] = drop ? 0 : 1;
+                    else if(drop)
+                        kp[c] = 0;
                 }
             }
-            double d = 0.0;
+        }
+        if(pass == 0 && ndropped == jb)
+        {
+            // the range is exhausted: the panel is zero, later panels are
+            // skipped.  Q2 is used in place (no compaction), so the
+            // columns of those panels are zeroed here and Q2^T Bt ignores
+            // them.
 #pragma unroll
             for(int r = 0; r < RPT; r++)
-                d += w[r][c]*w[r][c];
-            d = wave_sum(d);
-            if(lane == 0)
-                pred[rb][wave][0] = d;
-            __syncthreads();
-            double n2 = 0.0;
+            {
+                const int i = threadIdx.x+r*NT;
+                if(i < rows)
 #pragma unroll
-            for(int q2w = 0; q2w < NW; q2w++)
-                n2 += pred[rb][q2w][0];
-            rb ^= 1;
-            const double r2 = ref ? ref[(int64_t)blockIdx.x*kmax+j0+c] : 1.0;
-            const bool drop = !(n2 > tau2*r2) || n2 == 0.0;
-            const double inv = drop ? 0.0 : 1.0/sqrt(n2);
-#pragma unroll
-            for(int r = 0; r < RPT; r++)
-                w[r][c] *= inv;
-            if(drop)
-                ndropped++;
+                    for(int c = 0; c < PB; c++)
+                        if(c < jb)
+                            W[(int64_t)c*rows+i] = 0.0;
+            }
+            if(q2)
+            {
+                double *rest = W+(int64_t)jb*rows;
+                const int64_t cnt = (int64_t)(width-j0-jb)*rows;
+                for(int64_t e = threadIdx.x; e < cnt; e += NT)
+                    rest[e] = 0.0;
+            }
             if(threadIdx.x == 0)
             {
-                if(first_pass)
-                    kp[c] = drop ? 0 : 1;
-                else if(drop)
-                    kp[c] = 0;
+                td.flags |= done_bit;
+                atomicSub(active, 1);
             }
+            return;
         }
     }
 #pragma unroll
@@ -283,26 +320,8 @@ __global__ __launch_bounds__(NT) void bcgs_kernel(TileDesc *tiles, int which,
                 if(c < jb)
                     W[(int64_t)c*rows+i] = w[r][c];
     }
-    if(q2 && first_pass && ndropped == jb)
-    {
-        // Q2 is used in place (no compaction): the columns of the panels
-        // that will be skipped are zeroed here, so Q2^T Bt ignores them.
-        double *rest = W+(int64_t)jb*rows;
-        const int64_t cnt = (int64_t)(width-j0-jb)*rows;
-        for(int64_t e = threadIdx.x; e < cnt; e += NT)
-            rest[e] = 0.0;
-    }
-    if(threadIdx.x == 0)
-    {
-        const bool last = j0+jb >= width;
-        if(first_pass && ndropped == jb)
-        {
-            td.flags |= done_bit;
-            atomicSub(active, 1);
-        }
-        else if(!first_pass && last)
-            atomicSub(active, 1);   // finished all panels
-    }
+    if(threadIdx.x == 0 && j0+jb >= width)
+        atomicSub(active, 1);   // finished all panels
 }
 
 // Moves kept columns of Q to the front, in order; sets kc.
@@ -420,17 +439,17 @@ int pipeline_count_active(const TileDesc *d_tiles, int ntiles, int which,
 
 int pipeline_panel(const TileDesc *d_tiles, int ntiles, int which,
         double *ws, int j0, int jb, const double *d_ref, int kmax,
-        double tau, int *d_keep, int rows_max, int *d_active, hipStream_t s)
+        double tau_first, double tau_second, int *d_keep, int rows_max,
+        int *d_active, hipStream_t s)
 {
     if(ntiles <= 0)
         return STARSH_SUCCESS;
     TileDesc *tiles = const_cast<TileDesc *>(d_tiles);
-    const int first = d_ref != nullptr;
-    const double tau2 = tau*tau;
+    const double tau2a = tau_first*tau_first, tau2b = tau_second*tau_second;
 #define BCGS_LAUNCH(NT, RPT, PB) \
         hipLaunchKernelGGL((bcgs_kernel<NT, RPT, PB>), dim3(ntiles), \
-                dim3(NT), 0, s, tiles, which, ws, j0, d_ref, kmax, tau2, \
-                d_keep, first, d_active)
+                dim3(NT), 0, s, tiles, which, ws, j0, d_ref, kmax, tau2a, \
+                tau2b, d_keep, d_active)
     if(jb == 4 && rows_max <= 1024)
         BCGS_LAUNCH(256, 4, 4);
     else if(jb == 4 && rows_max <= 2048)
