@@ -213,35 +213,37 @@ __device__ inline double kval_pt(const KernelParams &kp,
     return kval_of_dist<FAM>(kp, dist);
 }
 
-// exp(x) for finite x <= 0 to within about two units in the last place, by
-// the table-driven method of P. T. P. Tang ("Table-driven implementation of
-// the exponential function in IEEE floating-point arithmetic", ACM TOMS
-// 15(2), 1989): x = (32 N + j) ln2/32 + r with |r| <= ln2/64, so exp(x) =
-// 2^N 2^(j/32) e^r, with e^r - 1 from its Taylor polynomial of degree 6
-// (remainder below 2^-58).  tab[j] = 2^(j/32), j = 0..31 (exp2_table).
-// Arguments below -1000 give 0, like exp.
+// exp(x) for finite x <= 0 to a relative error below 2^-43, enough for the
+// int8 paths, which round every entry to 2^-31 of its chunk's largest one.
+// Table-driven method of P. T. P. Tang ("Table-driven implementation of the
+// exponential function in IEEE floating-point arithmetic", ACM TOMS 15(2),
+// 1989): x = (64 N + j) ln2/64 + r with |r| <= ln2/128, so exp(x) =
+// 2^N 2^(j/64) e^r, with e^r - 1 from its Taylor polynomial of degree 4
+// (remainder below 2^-44).  The nearest integer to 64 x/ln2 is read off the
+// low word of 1.5 2^52 + 64 x/ln2.  tab[j] = 2^(j/64), j = 0..63
+// (exp2_table).  Arguments below -1000 give 0, like exp.
 __device__ inline double exp_nonpos(double x, const double *tab)
 {
+    constexpr double shift = 0x1.8p52;
     x = x > -1000.0 ? x : -1000.0;
-    const double nd = rint(x*46.16624130844683);        // 32/ln2
-    const int n = (int)nd;
-    double r = fma(-nd, 0.02166084939249829, x);        // ln2/32, high part
-    r = fma(-nd, 7.247021293269686e-19, r);             // low part
-    double p = fma(r, 1.0/720.0, 1.0/120.0);
-    p = fma(p, r, 1.0/24.0);
-    p = fma(p, r, 1.0/6.0);
+    const double sh = fma(x, 92.33248261689366, shift);    // 64/ln2
+    const int n = (int)__double_as_longlong(sh);
+    const double nd = sh-shift;
+    double r = fma(-nd, 0.010830424696249145, x);        // ln2/64, high part
+    r = fma(-nd, 3.623510646634843e-19, r);              // low part
+    double p = fma(r, 1.0/24.0, 1.0/6.0);
     p = fma(p, r, 0.5);
     p = fma(p, r, 1.0);
     p *= r;
-    const double t = tab[n & 31];
-    return ldexp(fma(t, p, t), n >> 5);
+    const double t = tab[n & 63];
+    return ldexp(fma(t, p, t), n >> 6);
 }
 
-// Fills tab[0..31] with 2^(j/32); called by the first 32 threads.
+// Fills tab[0..63] with 2^(j/64); called by the first 64 threads.
 __device__ inline void exp2_table(double *tab, int tid)
 {
-    if(tid < 32)
-        tab[tid] = exp2((double)tid*(1.0/32.0));
+    if(tid < 64)
+        tab[tid] = exp2((double)tid*(1.0/64.0));
 }
 
 // kval_of_dist with exp_nonpos for the exponential families (exp, sqexp,
@@ -326,6 +328,30 @@ __device__ inline void byte_planes(unsigned z0, unsigned z1, unsigned z2,
 __device__ inline unsigned byte_digits(int x)
 {
     return ((unsigned)x+0x808080u) ^ 0x808080u;
+}
+
+// Scale of a chunk of entries whose largest magnitude is mx = f 2^ex (f in
+// [0.5, 1)): 2^(30-ex) maps the chunk onto [-2^30, 2^30].  A chunk below
+// 2^-990 (or zero) gets 0, so its entries are left out.
+__device__ inline double digit_scale(double mx, int ex)
+{
+    return mx > 0.0 && ex >= -990 ? ldexp(1.0, 30-ex) : 0.0;
+}
+
+// Inverse of digit_scale: the weight 2^(ex-30) of one unit, 0 for a chunk
+// left out.
+__device__ inline double digit_unit(double mx, int ex)
+{
+    return mx > 0.0 && ex >= -990 ? ldexp(1.0, ex-30) : 0.0;
+}
+
+// byte_digits(rint(a * scale)) for |a * scale| <= 2^30: the low word of the
+// double 1.5 2^52 + 0x808080 + a * scale (one rounding, to the nearest
+// integer) is rint(a * scale) + 0x808080.
+__device__ inline unsigned scaled_byte_digits(double a, double scale)
+{
+    const double sh = fma(a, scale, 0x1.8p52+8421504.0);
+    return (unsigned)__double_as_longlong(sh) ^ 0x808080u;
 }
 
 // Workgroups are dispatched to the 8 XCDs of the chip round robin (group

@@ -29,7 +29,7 @@ __global__ __launch_bounds__(256) void qdigits_kernel(const TileDesc *tiles,
         const double *ws, signed char *qd)
 {
     __shared__ double red[4];
-    __shared__ int sh_ex;
+    __shared__ double sh_scale;
     const TileDesc t = tiles[blockIdx.x];
     const int c = blockIdx.y;
     if(c >= t.kc || t.m <= 0)
@@ -51,12 +51,12 @@ __global__ __launch_bounds__(256) void qdigits_kernel(const TileDesc *tiles,
         int ex = 0;
         if(v > 0.0)
             frexp(v, &ex);      // v = f 2^ex, f in [0.5, 1)
-        sh_ex = ex;
         double *scale = (double *)(qd+t.qd_off+4*(int64_t)k16*m64);
-        scale[c] = v > 0.0 ? ldexp(1.0, ex-30) : 0.0;
+        scale[c] = digit_unit(v, ex);
+        sh_scale = digit_scale(v, ex);
     }
     __syncthreads();
-    const int ex = sh_ex;
+    const double dsc = sh_scale;
     signed char *base = qd+t.qd_off;
     // four consecutive rows per thread and step
     for(int i0 = 4*tid; i0 < m64; i0 += 1024)
@@ -66,8 +66,7 @@ __global__ __launch_bounds__(256) void qdigits_kernel(const TileDesc *tiles,
         for(int e = 0; e < 4; e++)
         {
             const int i = i0+e;
-            const double v = i < t.m ? q[i] : 0.0;
-            z[e] = byte_digits((int)rint(ldexp(v, 30-ex)));
+            z[e] = scaled_byte_digits(i < t.m ? q[i] : 0.0, dsc);
         }
         unsigned pl[4];
         byte_planes(z[0], z[1], z[2], z[3], pl);
@@ -101,7 +100,7 @@ void proj_i8_kernel(const TileDesc *tiles, double *ws,
 {
     constexpr int NF = CB/16;
     __shared__ __align__(16) double xs[4][2][NDIM][KB];
-    __shared__ double tab[32];
+    __shared__ double tab[64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     exp2_table(tab, tid);
     __syncthreads();
@@ -179,6 +178,7 @@ void proj_i8_kernel(const TileDesc *tiles, double *ws,
         int ex = 0;
         if(mx > 0.0)
             frexp(mx, &ex);
+        const double dsc = digit_scale(mx, ex);
         int4v as[4];
 #pragma unroll
         for(int g = 0; g < 4; g++)
@@ -186,7 +186,7 @@ void proj_i8_kernel(const TileDesc *tiles, double *ws,
             unsigned z[4];
 #pragma unroll
             for(int e = 0; e < 4; e++)
-                z[e] = byte_digits((int)rint(ldexp(a[4*g+e], 30-ex)));
+                z[e] = scaled_byte_digits(a[4*g+e], dsc);
             unsigned pl[4];
             byte_planes(z[0], z[1], z[2], z[3], pl);
 #pragma unroll
@@ -198,7 +198,7 @@ void proj_i8_kernel(const TileDesc *tiles, double *ws,
             xs[wave][buf^1][d][lane] = xn[d];
         // scale of output row 4(l/16) + v (held by the lanes with
         // l%16 = that row); 2^16 for the weight of the lowest group kept
-        const double sa = mx > 0.0 ? ldexp(1.0, ex-14) : 0.0;
+        const double sa = digit_unit(mx, ex)*65536.0;
         double sc[4];
 #pragma unroll
         for(int v = 0; v < 4; v++)

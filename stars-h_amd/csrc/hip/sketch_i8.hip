@@ -86,7 +86,7 @@ void sketch_i8_kernel(const TileDesc *tiles,
     __shared__ __align__(16) signed char As[4][4][16*APAD];
     __shared__ __align__(16) double xs[4][2][NDIM][KB];
     __shared__ double csc[4][16];
-    __shared__ double tab[32];
+    __shared__ double tab[64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     exp2_table(tab, tid);
     __syncthreads();
@@ -167,16 +167,15 @@ void sketch_i8_kernel(const TileDesc *tiles,
         if(mx > 0.0)
             frexp(mx, &ex);     // mx = f 2^ex, f in [0.5, 1)
         if((lane & 3) == 0)
-            csc[wave][gi] = mx > 0.0 ? ldexp(1.0, ex-30) : 0.0;
+            csc[wave][gi] = digit_unit(mx, ex);
+        const double dsc = digit_scale(mx, ex);
 #pragma unroll
         for(int g = 0; g < 4; g++)
         {
             unsigned z[4];
 #pragma unroll
             for(int e = 0; e < 4; e++)
-            {
-                z[e] = byte_digits((int)rint(ldexp(a[4*g+e], 30-ex)));
-            }
+                z[e] = scaled_byte_digits(a[4*g+e], dsc);
             unsigned pl[4];
             byte_planes(z[0], z[1], z[2], z[3], pl);
 #pragma unroll

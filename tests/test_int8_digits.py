@@ -81,20 +81,22 @@ def _fma(a, b, c):
 
 def _exp_nonpos(x):
     """Numpy restatement of exp_nonpos (stars-h_amd/csrc/hip/hip_common.h):
-    Tang's table-driven exp with 32 table entries and a degree-6 Taylor
+    Tang's table-driven exp with 64 table entries and a degree-4 Taylor
     polynomial; fma emulated in extended precision."""
     x = np.maximum(x, -1000.0)
-    nd = np.rint(x * 46.16624130844683)
-    n = nd.astype(np.int64)
-    r = _fma(-nd, 0.02166084939249829, x)
-    r = _fma(-nd, 7.247021293269686e-19, r)
-    p = _fma(r, 1.0 / 720.0, 1.0 / 120.0)
-    for c in (1.0 / 24.0, 1.0 / 6.0, 0.5, 1.0):
+    shift = 1.5 * 2.0**52
+    sh = _fma(x, 92.33248261689366, shift)
+    n = sh.view(np.int64).astype(np.int32)         # low 32 bits
+    nd = sh - shift
+    r = _fma(-nd, 0.010830424696249145, x)
+    r = _fma(-nd, 3.623510646634843e-19, r)
+    p = _fma(r, 1.0 / 24.0, 1.0 / 6.0)
+    for c in (0.5, 1.0):
         p = _fma(p, r, c)
     p = p * r
-    tab = np.exp2(np.arange(32) / 32.0)
-    t = tab[n & 31]
-    return np.ldexp(_fma(t, p, t), (n >> 5).astype(np.int32))
+    tab = np.exp2(np.arange(64) / 64.0)
+    t = tab[n & 63]
+    return np.ldexp(_fma(t, p, t), (n >> 6).astype(np.int32))
 
 
 def test_table_exp_accuracy():
@@ -105,5 +107,27 @@ def test_table_exp_accuracy():
     want = np.exp(x.astype(np.longdouble))
     ok = want > 1e-300          # normal range
     rel = np.abs((got[ok] - want[ok]) / want[ok]).astype(np.float64)
-    assert rel.max() <= 4.5e-16
+    assert rel.max() <= 2.0**-43
     assert _exp_nonpos(np.array([-1e6, -1000.0]))[0] == 0.0
+
+
+def test_scaled_byte_digits_rounding():
+    """scaled_byte_digits: the low word of 1.5*2^52 + 0x808080 + a*scale
+    (a*scale exact for a power-of-two scale, so one rounding) equals
+    byte_digits(rint(a*scale)), ties to even, over [-2^30, 2^30]."""
+    rng = np.random.default_rng(4)
+    ex = rng.integers(-60, 60, 50000)
+    scale = np.ldexp(1.0, 30 - ex)
+    f = rng.uniform(-1.0, 1.0, 50000)
+    a = np.ldexp(f, ex)
+    a[:8] = np.ldexp(np.array([0.5, -0.5, 1.5, -1.5, 2.5, 0.0, 1.0, -1.0]),
+                     -30 + ex[:8])           # exact ties and extremes
+    sh = a * scale + (1.5 * 2.0**52 + 8421504.0)
+    low = (sh.view(np.int64) & 0xFFFFFFFF).astype(np.uint64)
+    z = (low ^ 0x808080).astype(np.int64)
+    want = np.rint(a * scale).astype(np.int64)
+    assert np.abs(want).max() <= 2**30
+    got = np.stack([(z >> (8 * s)) & 0xFF for s in range(4)])
+    got = np.where(got >= 128, got - 256, got)
+    back = got[3] * 2**24 + got[2] * 2**16 + got[1] * 2**8 + got[0]
+    np.testing.assert_array_equal(back, want)
