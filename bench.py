@@ -40,7 +40,7 @@ PEAK_FP64 = 78.6  # TFLOP/s, MI355X dense FP64 (vector and matrix)
 # FETCH_SIZE and WRITE_SIZE counters (rocprofv3 --pmc, one run per counter)
 # scaled by the same counters on a 1 GiB device copy (fetch x2.00, write
 # x1.00); commands and raw values in profiles/c2_sketch_hbm_bytes.md.
-C2_SKETCH_TRAFFIC = 7.346e9
+C2_SKETCH_TRAFFIC = 7.348e9
 
 WORKLOADS = {
     "C2": dict(kind="spatial", kernel="sqrexp", N=64000, ndim=2, nb=1600,
@@ -256,6 +256,11 @@ def main():
     if not args.no_cpu and plain is not None and w["kind"] != "batch":
         print("bench: CPU baseline sample", file=sys.stderr, flush=True)
         cpu = cpu_baseline(w, plain["problems"][0], args.cpu_sample)
+    # attainable FP64 rates of this GPU (MFMA chains, vector FMA)
+    mf, vf = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    probe = None
+    if S.lib().starsh_hip_fp64_probe(ctypes.byref(mf), ctypes.byref(vf)) == 0:
+        probe = {"mfma": round(mf.value, 2), "valu": round(vf.value, 2)}
     dom = plain or head
     dom_t = dom["dominant_kernel_seconds"]
     achieved = dom["sketch_flops"] / dom_t / 1e12 if dom_t > 0 else 0.0
@@ -288,9 +293,12 @@ def main():
                      "bound": "mfma", "achieved": round(achieved, 3),
                      "peak": PEAK_FP64, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64, 4),
+                     "frac_of_attainable": (round(achieved / probe["mfma"], 4)
+                                            if probe else None),
                      "traffic": (C2_SKETCH_TRAFFIC if args.config == "C2"
                                  and plain is not None else None)},
         "cpu_baseline": cpu,
+        "fp64_attainable_tflops": probe,
     }
     for r in forms.values():
         for p in r["problems"]:

@@ -91,6 +91,12 @@ typedef struct starsh_hip_stats
 
 int starsh_hip_get_stats(starsh_hip_stats *stats);
 
+/* Measured FP64 rates of the current GPU in TFLOP/s (no reference
+ * counterpart): independent v_mfma_f64_16x16x4f64 chains on every SIMD,
+ * and FP64 fused multiply-adds on the vector ALUs.  For reading the
+ * roofline fractions of the FP64 kernels against what the chip sustains. */
+int starsh_hip_fp64_probe(double *mfma_tflops, double *valu_tflops);
+
 /* Tuning options of the HIP drivers (no reference counterpart):
  *   "workspace_gb"        device workspace budget per batch (default 96);
  *   "mixed_first_bucket"  first rank bucket of the mixed form, an index into
