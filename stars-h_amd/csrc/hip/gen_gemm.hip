@@ -195,12 +195,16 @@ __global__ __launch_bounds__(256) void gen_gemm_kernel(const TileDesc *tiles,
 #pragma unroll
                 for(int f = 0; f < NF; f++)
                 {
-                    // each B value feeds RW MFMAs
-                    const double b = Bs[buf][4*q+kl][f*16+(lane & 15)];
+                    // fragments past the tile's own width are skipped
+                    // (uniform); each B value feeds RW MFMAs
+                    if(!PROJ || f < nf_live)
+                    {
+                        const double b = Bs[buf][4*q+kl][f*16+(lane & 15)];
 #pragma unroll
-                    for(int r = 0; r < RW; r++)
-                        acc[r][f] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                                acur[r][q], b, acc[r][f], 0, 0, 0);
+                        for(int r = 0; r < RW; r++)
+                            acc[r][f] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                    acur[r][q], b, acc[r][f], 0, 0, 0);
+                    }
                 }
             }
         }
