@@ -261,6 +261,17 @@ def main():
     probe = None
     if S.lib().starsh_hip_fp64_probe(ctypes.byref(mf), ctypes.byref(vf)) == 0:
         probe = {"mfma": round(mf.value, 2), "valu": round(vf.value, 2)}
+    # product with the compressed matrix (starsh_hip_tlr_*, factors
+    # resident in HBM): one right-hand side, 20 products
+    matvec = None
+    if w["kind"] != "batch":
+        sec, nbytes = head["problems"][0].tlr_bench(1, 20)
+        if sec > 0:
+            matvec = {"form": head_name, "nrhs": 1,
+                      "seconds": round(sec, 7),
+                      "bytes": nbytes,
+                      "GB_per_s": round(nbytes / sec / 1e9, 1),
+                      "frac_of_hbm_peak": round(nbytes / sec / 8e12, 4)}
     dom = plain or head
     dom_t = dom["dominant_kernel_seconds"]
     achieved = dom["sketch_flops"] / dom_t / 1e12 if dom_t > 0 else 0.0
@@ -299,6 +310,7 @@ def main():
                                  and plain is not None else None)},
         "cpu_baseline": cpu,
         "fp64_attainable_tflops": probe,
+        "tlr_matvec": matvec,
     }
     for r in forms.values():
         for p in r["problems"]:

@@ -206,3 +206,66 @@ def zsort(points):
     order = np.empty(count, dtype=np.int64)
     lib().oracle_zsort(C.c_int64(count), C.c_int(ndim), _d(pts), _l(order))
     return pts, order
+
+
+def dmml(starts, sizes, symm, far_tiles, Us, Vs, near_tiles, Ds, A, alpha,
+         beta, B, col_starts=None):
+    """B = alpha * M * A + beta * B for a compressed matrix M, restating
+    starsh_blrm__dmml (src/backends/sequential/blrm/dmml.c:16-145): B is
+    scaled first (set to zero when beta == 0, :52-59); far tile (i, j) adds
+    alpha U (V^T A_j) to B_i and, when symm == 'S' and i != j, alpha V (U^T
+    A_i) to B_j (:61-91); near tile (i, j) adds alpha D A_j and, symmetric
+    with i != j, alpha D^T A_i (:92-145).  starts/sizes describe the row
+    blocks, col_starts (default: the same) the column blocks; A and B are
+    (rows x nrhs) arrays in cluster order.  Returns the new B."""
+    cs = starts if col_starts is None else col_starts
+    B = np.zeros_like(B) if beta == 0.0 else beta * B
+    A = np.asarray(A, dtype=np.float64)
+    for (i, j), U, V in zip(far_tiles, Us, Vs):
+        ri = slice(starts[i], starts[i] + U.shape[0])
+        cj = slice(cs[j], cs[j] + V.shape[0])
+        if U.shape[1] > 0:
+            B[ri] += alpha * (U @ (V.T @ A[cj]))
+            if symm == "S" and i != j:
+                B[cj] += alpha * (V @ (U.T @ A[ri]))
+    for (i, j), D in zip(near_tiles, Ds):
+        ri = slice(starts[i], starts[i] + D.shape[0])
+        cj = slice(cs[j], cs[j] + D.shape[1])
+        B[ri] += alpha * (D @ A[cj])
+        if symm == "S" and i != j:
+            B[cj] += alpha * (D.T @ A[ri])
+    return B
+
+
+def cg(matvec, B, X, tol):
+    """Conjugate gradients restating starsh_itersolvers__dcg_omp
+    (src/itersolvers/cg.c:24-95): R = B - M X, P = R; column q stops once
+    |r_q| < tol |r_q(0)|; returns (iteration at which every column stopped,
+    or -1 after n iterations, X).  matvec(P) returns M P."""
+    X = np.array(X, dtype=np.float64)
+    n, nrhs = B.shape
+    R = B - matvec(X)
+    P = R.copy()
+    rsold = np.linalg.norm(R, axis=0)
+    rscheck = rsold * tol
+    rsold = rsold ** 2
+    finished = 0
+    for it in range(n):
+        Q = matvec(P)
+        for q in range(nrhs):
+            if rscheck[q] < 0:
+                continue
+            a = rsold[q] / (P[:, q] @ Q[:, q])
+            X[:, q] += a * P[:, q]
+            R[:, q] -= a * Q[:, q]
+            rsnew = np.linalg.norm(R[:, q])
+            if rsnew < rscheck[q]:
+                finished += 1
+                rscheck[q] = -1.0
+                continue
+            rsnew *= rsnew
+            P[:, q] = R[:, q] + (rsnew / rsold[q]) * P[:, q]
+            rsold[q] = rsnew
+        if finished == nrhs:
+            return it, X
+    return -1, X

@@ -1,0 +1,280 @@
+// Product of a compressed (tile low-rank) matrix with a dense block of
+// vectors on the GPU, Y = alpha * M * X + beta * Y, and the vector kernels
+// of conjugate gradients.  Follows src/backends/sequential/blrm/dmml.c:16-145
+// of ecrc/stars-h 0.3.1: every far-field tile (i, j) adds U (V^T X_j) to Y_i
+// and, in a symmetric format with i != j, V (U^T X_i) to Y_j; near-field
+// tiles add D X_j (and D^T X_i).
+//
+// Two launches, both deterministic (no atomics):
+//  1. tlr_coef_kernel: per far-field tile and direction, the r x nrhs
+//     coefficients T = V^T X_j (and T' = U^T X_i); one workgroup per tile
+//     and direction, wave w takes factor columns w, w+4, ...;
+//  2. tlr_rows_kernel: per chunk of 64 rows of one block row, the sum of all
+//     contributions to those rows (U T, V T', D X_j, D^T X_i) in a fixed
+//     order: lane = row, wave w takes entries w, w+4, ... of the block row,
+//     and the four partial sums are added in LDS in wave order.
+#include "hip_common.h"
+#include "tlr.h"
+
+namespace {
+
+constexpr int RHS_GROUP = 4;
+
+__global__ __launch_bounds__(256) void tlr_coef_kernel(const tlr_tile *tiles,
+        const double *fac, const double *x, int ldx, int nrhs, double *tmp)
+{
+    const tlr_tile t = tiles[blockIdx.x];
+    const int dir = blockIdx.y;
+    if(dir == 1 && !t.twin)
+        return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // dir 0: T = V^T X[c0..c0+n); dir 1: T' = U^T X[r0..r0+m)
+    const double *F = fac+(dir == 0 ? t.v_off : t.u_off);
+    const int len = dir == 0 ? t.n : t.m;
+    const int64_t x0 = dir == 0 ? t.c0 : t.r0;
+    double *T = tmp+(t.tmp_off+(int64_t)dir*t.rank)*nrhs;
+    for(int c = wave; c < t.rank; c += 4)
+    {
+        const double *col = F+(int64_t)c*len;
+        for(int q = 0; q < nrhs; q++)
+        {
+            const double *xv = x+(int64_t)q*ldx+x0;
+            double s = 0.0;
+            for(int l = lane; l < len; l += 64)
+                s = fma(col[l], xv[l], s);
+#pragma unroll
+            for(int o = 32; o > 0; o >>= 1)
+                s += __shfl_xor(s, o, 64);
+            if(lane == 0)
+                T[c+(int64_t)q*t.rank] = s;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void tlr_rows_kernel(const tlr_chunk *chunks,
+        const tlr_entry *entries, const double *fac, const double *tmp,
+        const double *x, int ldx, int nrhs, double alpha, double beta,
+        double *y, int ldy)
+{
+    __shared__ double part[4][RHS_GROUP][64];
+    const tlr_chunk ch = chunks[blockIdx.x];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool row_ok = lane < ch.nrows;
+    for(int q0 = 0; q0 < nrhs; q0 += RHS_GROUP)
+    {
+        const int nq = nrhs-q0 < RHS_GROUP ? nrhs-q0 : RHS_GROUP;
+        double acc[RHS_GROUP] = {0.0, 0.0, 0.0, 0.0};
+        for(int e = ch.first+wave; e < ch.last; e += 4)
+        {
+            const tlr_entry en = entries[e];
+            if(!row_ok)
+                continue;
+            if(en.kind == 0)
+            {
+                const double *F = fac+en.f_off+ch.rofs+lane;
+                const double *T = tmp+en.src_off*nrhs;
+                for(int c = 0; c < en.cols; c++)
+                {
+                    const double f = F[(int64_t)c*en.ld];
+#pragma unroll
+                    for(int q = 0; q < RHS_GROUP; q++)
+                        if(q < nq)
+                            acc[q] = fma(f, T[c+(int64_t)(q0+q)*en.cols],
+                                    acc[q]);
+                }
+            }
+            else
+            {
+                const double *D = fac+en.f_off;
+                for(int c = 0; c < en.cols; c++)
+                {
+                    const int r = ch.rofs+lane;
+                    const double f = en.kind == 1 ?
+                        D[r+(int64_t)c*en.ld] : D[c+(int64_t)r*en.ld];
+#pragma unroll
+                    for(int q = 0; q < RHS_GROUP; q++)
+                        if(q < nq)
+                            acc[q] = fma(f, x[en.src_off+c+
+                                    (int64_t)(q0+q)*ldx], acc[q]);
+                }
+            }
+        }
+#pragma unroll
+        for(int q = 0; q < RHS_GROUP; q++)
+            part[wave][q][lane] = acc[q];
+        __syncthreads();
+        if(wave == 0 && row_ok)
+        {
+            for(int q = 0; q < nq; q++)
+            {
+                const double s = ((part[0][q][lane]+part[1][q][lane])+
+                        part[2][q][lane])+part[3][q][lane];
+                double *yy = y+(int64_t)(q0+q)*ldy+ch.row0+lane;
+                *yy = beta == 0.0 ? alpha*s : fma(alpha, s, beta*(*yy));
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---- vector kernels of conjugate gradients (one column of length n)
+
+constexpr int DOT_BLOCKS = 256;
+
+// part[b] = sum over the block's grid-stride share of a[i]*b[i]
+__global__ __launch_bounds__(256) void dot_kernel(const double *a,
+        const double *b, int64_t n, double *part)
+{
+    __shared__ double red[4];
+    double s = 0.0;
+    for(int64_t i = (int64_t)blockIdx.x*256+threadIdx.x; i < n;
+            i += (int64_t)gridDim.x*256)
+        s = fma(a[i], b[i], s);
+#pragma unroll
+    for(int o = 32; o > 0; o >>= 1)
+        s += __shfl_xor(s, o, 64);
+    if((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if(threadIdx.x == 0)
+        part[blockIdx.x] = ((red[0]+red[1])+red[2])+red[3];
+}
+
+// out[0] = sum of part[0..np) in index order
+__global__ void sum_kernel(const double *part, int np, double *out)
+{
+    if(threadIdx.x == 0 && blockIdx.x == 0)
+    {
+        double s = 0.0;
+        for(int i = 0; i < np; i++)
+            s += part[i];
+        out[0] = s;
+    }
+}
+
+// y = a*x + y
+__global__ __launch_bounds__(256) void axpy_kernel(int64_t n, double a,
+        const double *x, double *y)
+{
+    for(int64_t i = (int64_t)blockIdx.x*256+threadIdx.x; i < n;
+            i += (int64_t)gridDim.x*256)
+        y[i] = fma(a, x[i], y[i]);
+}
+
+// y = x
+__global__ __launch_bounds__(256) void copy_kernel(int64_t n, const double *x,
+        double *y)
+{
+    for(int64_t i = (int64_t)blockIdx.x*256+threadIdx.x; i < n;
+            i += (int64_t)gridDim.x*256)
+        y[i] = x[i];
+}
+
+// p = r + b*p
+__global__ __launch_bounds__(256) void xpby_kernel(int64_t n,
+        const double *r, double b, double *p)
+{
+    for(int64_t i = (int64_t)blockIdx.x*256+threadIdx.x; i < n;
+            i += (int64_t)gridDim.x*256)
+        p[i] = fma(b, p[i], r[i]);
+}
+
+inline unsigned vec_grid(int64_t n)
+{
+    const int64_t g = (n+255)/256;
+    return (unsigned)(g < 1024 ? (g > 0 ? g : 1) : 1024);
+}
+
+} // namespace
+
+
+int starsh_hip_tlr_apply(const tlr_tile *d_tiles, int ntiles,
+        const tlr_chunk *d_chunks, int nchunks, const tlr_entry *d_entries,
+        const double *d_fac, double *d_tmp, int nrhs, double alpha,
+        const double *d_x, int ldx, double beta, double *d_y, int ldy)
+{
+    if(nrhs <= 0)
+        return STARSH_SUCCESS;
+    hipStream_t s = starsh_hip_stream();
+    if(ntiles > 0)
+    {
+        hipLaunchKernelGGL(tlr_coef_kernel, dim3(ntiles, 2), dim3(256), 0,
+                s, d_tiles, d_fac, d_x, ldx, nrhs, d_tmp);
+        HIP_TRY(hipGetLastError());
+    }
+    if(nchunks > 0)
+    {
+        hipLaunchKernelGGL(tlr_rows_kernel, dim3(nchunks), dim3(256), 0, s,
+                d_chunks, d_entries, d_fac, d_tmp, d_x, ldx, nrhs, alpha, beta, d_y, ldy);
+        HIP_TRY(hipGetLastError());
+    }
+    return STARSH_SUCCESS;
+}
+
+// *d_out = sum a[i]*b[i] (device scalar; d_part holds DOT_BLOCKS doubles)
+int starsh_hip_vdot(int64_t n, const double *d_a, const double *d_b,
+        double *d_part, double *d_out)
+{
+    hipStream_t s = starsh_hip_stream();
+    hipLaunchKernelGGL(dot_kernel, dim3(DOT_BLOCKS), dim3(256), 0, s, d_a,
+            d_b, n, d_part);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(64), 0, s, d_part,
+            DOT_BLOCKS, d_out);
+    HIP_TRY(hipGetLastError());
+    return STARSH_SUCCESS;
+}
+
+int starsh_hip_vaxpy(int64_t n, double a, const double *d_x, double *d_y)
+{
+    hipLaunchKernelGGL(axpy_kernel, dim3(vec_grid(n)), dim3(256), 0,
+            starsh_hip_stream(), n, a, d_x, d_y);
+    HIP_TRY(hipGetLastError());
+    return STARSH_SUCCESS;
+}
+
+int starsh_hip_vcopy(int64_t n, const double *d_x, double *d_y)
+{
+    hipLaunchKernelGGL(copy_kernel, dim3(vec_grid(n)), dim3(256), 0,
+            starsh_hip_stream(), n, d_x, d_y);
+    HIP_TRY(hipGetLastError());
+    return STARSH_SUCCESS;
+}
+
+int starsh_hip_vxpby(int64_t n, const double *d_r, double b, double *d_p)
+{
+    hipLaunchKernelGGL(xpby_kernel, dim3(vec_grid(n)), dim3(256), 0,
+            starsh_hip_stream(), n, d_r, b, d_p);
+    HIP_TRY(hipGetLastError());
+    return STARSH_SUCCESS;
+}
+
+// Seconds per starsh_hip_tlr_apply (mean over reps launches after one
+// warm-up), measured with events on the library stream.
+int starsh_hip_tlr_time(const tlr_tile *d_tiles, int ntiles,
+        const tlr_chunk *d_chunks, int nchunks, const tlr_entry *d_entries,
+        const double *d_fac, double *d_tmp, int nrhs, const double *d_x,
+        int ldx, double *d_y, int ldy, int reps, double *seconds)
+{
+    hipStream_t s = starsh_hip_stream();
+    int info = starsh_hip_tlr_apply(d_tiles, ntiles, d_chunks, nchunks,
+            d_entries, d_fac, d_tmp, nrhs, 1.0, d_x, ldx, 0.0, d_y, ldy);
+    if(info != STARSH_SUCCESS)
+        return info;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s));
+    for(int r = 0; r < reps && info == STARSH_SUCCESS; r++)
+        info = starsh_hip_tlr_apply(d_tiles, ntiles, d_chunks, nchunks,
+                d_entries, d_fac, d_tmp, nrhs, 1.0, d_x, ldx, 0.0, d_y, ldy);
+    HIP_TRY(hipEventRecord(e1, s));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *seconds = reps > 0 ? ms*1e-3/reps : 0.0;
+    return info;
+}
+

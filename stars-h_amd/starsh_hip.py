@@ -146,6 +146,18 @@ def lib():
         L.starsh_blrm__dfe_hip_tiles.restype = C.c_double
         L.starsh_dense_dsvfr.argtypes = [C.c_int, C.POINTER(C.c_double),
                                          C.c_double]
+        dp = C.POINTER(C.c_double)
+        L.starsh_blrm__dmml_hip.argtypes = [C.c_void_p, C.c_int, C.c_double,
+                                            dp, C.c_int, C.c_double, dp,
+                                            C.c_int]
+        L.starsh_itersolvers__dcg_hip.argtypes = [C.c_void_p, C.c_int, dp,
+                                                  C.c_int, dp, C.c_int,
+                                                  C.c_double, dp]
+        L.starsh_hip_tlr_new.argtypes = [C.POINTER(C.c_void_p), C.c_void_p]
+        L.starsh_hip_tlr_bench.argtypes = [C.c_void_p, C.c_int, C.c_int, dp,
+                                           dp]
+        L.starsh_hip_tlr_free.argtypes = [C.c_void_p]
+        L.starsh_hip_tlr_free.restype = None
         _lib = L
     return _lib
 
@@ -425,6 +437,47 @@ class TLRProblem:
                                     shape=(m * n,)).reshape(n, m).T.copy()
         C.CDLL(None).free(D)
         return arr
+
+    def dmml(self, A, alpha=1.0, beta=0.0, B=None):
+        """starsh_blrm__dmml_hip: returns alpha * M * A + beta * B (A, B:
+        rows x nrhs arrays in cluster order)."""
+        A = np.asfortranarray(A, dtype=np.float64)
+        n, nrhs = A.shape
+        B = (np.zeros((n, nrhs), order="F") if B is None else
+             np.array(B, dtype=np.float64, order="F"))
+        dp = C.POINTER(C.c_double)
+        info = self.L.starsh_blrm__dmml_hip(
+            C.cast(self.M, C.c_void_p), nrhs, alpha, A.ctypes.data_as(dp), n,
+            beta, B.ctypes.data_as(dp), n)
+        assert info == 0, info
+        return B
+
+    def cg(self, B, X0, tol):
+        """starsh_itersolvers__dcg_hip: returns (iteration, X)."""
+        B = np.asfortranarray(B, dtype=np.float64)
+        X = np.array(X0, dtype=np.float64, order="F")
+        n, nrhs = B.shape
+        work = np.zeros(3 * n * nrhs + 3 * nrhs)
+        dp = C.POINTER(C.c_double)
+        it = self.L.starsh_itersolvers__dcg_hip(
+            C.cast(self.M, C.c_void_p), nrhs, B.ctypes.data_as(dp), n,
+            X.ctypes.data_as(dp), n, tol, work.ctypes.data_as(dp))
+        return it, X
+
+    def tlr_bench(self, nrhs=1, reps=10):
+        """Seconds per device-resident product with nrhs columns and the
+        bytes it must move at least (starsh_hip_tlr_bench)."""
+        T = C.c_void_p()
+        assert self.L.starsh_hip_tlr_new(C.byref(T),
+                                         C.cast(self.M, C.c_void_p)) == 0
+        sec, nbytes = C.c_double(0.0), C.c_double(0.0)
+        try:
+            info = self.L.starsh_hip_tlr_bench(T, nrhs, reps, C.byref(sec),
+                                               C.byref(nbytes))
+            assert info == 0, info
+        finally:
+            self.L.starsh_hip_tlr_free(T)
+        return sec.value, nbytes.value
 
     def free_matrix(self):
         if self.M:

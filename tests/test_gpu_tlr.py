@@ -1,0 +1,103 @@
+"""Products with the compressed matrix (starsh_blrm__dmml_hip) and
+conjugate gradients (starsh_itersolvers__dcg_hip) on the GPU, against the
+oracle's restatements of src/backends/sequential/blrm/dmml.c and
+src/itersolvers/cg.c applied to the same factors.
+
+Cases: a symmetric format with false-far (near-field, stored) diagonal
+tiles, several right-hand sides with alpha and beta; a nonsymmetric format
+with near tiles generated on the fly (onfly = 1); CG on an SPD covariance
+with a nugget, compared with the oracle's CG on the same operator.
+"""
+import numpy as np
+import pytest
+
+import starsh_hip as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _operator(p, onfly):
+    st, sz = p.cluster_ranges()
+    far = p.tiles()
+    UV = [p.factors(b) for b in range(len(far))]
+    near = p.near_tiles()
+    if onfly:
+        Ds = [p.host_block(i, j) for (i, j) in near]
+    else:
+        Ds = [p.near_block(b) for b in range(len(near))]
+    return st, sz, far, [u for u, _ in UV], [v for _, v in UV], near, Ds
+
+
+def test_dmml_symmetric_with_stored_near_tiles():
+    _say("case: dmml, 2D exp N=2500 nb=250 maxrank 100 tol 1e-9, symm S")
+    fn = S.init("HIP", oversample=10)
+    p = S.TLRProblem("spatial", 2500, 2, 250, symm="S", seed=0,
+                     kernel="exp", beta=0.1)
+    assert p.approximate(100, 1e-9, 0, fn) == 0
+    st, sz, far, Us, Vs, near, Ds = _operator(p, 0)
+    assert len(near) > 0            # false-far diagonal tiles
+    rng = np.random.default_rng(0)
+    n = 2500
+    A = rng.standard_normal((n, 3))
+    B = rng.standard_normal((n, 3))
+    got = p.dmml(A, 0.7, 0.3, B)
+    want = O.dmml(st, sz, "S", far, Us, Vs, near, Ds, A, 0.7, 0.3, B)
+    rel = np.linalg.norm(got - want) / np.linalg.norm(want)
+    _say(f"  {len(far)} far, {len(near)} near tiles; rel diff {rel:.2e}")
+    assert rel <= 1e-13
+    got1 = p.dmml(A[:, :1])
+    want1 = O.dmml(st, sz, "S", far, Us, Vs, near, Ds, A[:, :1], 1.0, 0.0,
+                   np.zeros((n, 1)))
+    assert np.linalg.norm(got1 - want1) <= 1e-13 * np.linalg.norm(want1)
+    p.free_matrix()
+
+
+def test_dmml_nonsymmetric_near_tiles_on_the_fly():
+    _say("case: dmml, 2D exp N=1800 nb=300 maxrank 60 tol 1e-9, symm N, "
+         "onfly 1")
+    fn = S.init("HIP", oversample=10)
+    p = S.TLRProblem("spatial", 1800, 2, 300, symm="N", seed=3,
+                     kernel="exp", beta=0.1)
+    assert p.approximate(60, 1e-9, 1, fn) == 0
+    st, sz, far, Us, Vs, near, Ds = _operator(p, 1)
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((1800, 2))
+    B = rng.standard_normal((1800, 2))
+    got = p.dmml(A, -1.5, 2.0, B)
+    want = O.dmml(st, sz, "N", far, Us, Vs, near, Ds, A, -1.5, 2.0, B)
+    rel = np.linalg.norm(got - want) / np.linalg.norm(want)
+    _say(f"  {len(far)} far, {len(near)} near tiles; rel diff {rel:.2e}")
+    assert rel <= 1e-13
+    p.free_matrix()
+
+
+def test_cg_matches_oracle_cg():
+    _say("case: CG, 2D exp + nugget 0.05, N=2500 nb=250, tol 1e-8")
+    fn = S.init("HIP", oversample=10)
+    p = S.TLRProblem("spatial", 2500, 2, 250, symm="S", seed=0,
+                     kernel="exp", beta=0.1, noise=0.05)
+    assert p.approximate(100, 1e-9, 0, fn) == 0
+    st, sz, far, Us, Vs, near, Ds = _operator(p, 0)
+    rng = np.random.default_rng(2)
+    n = 2500
+    B = rng.standard_normal((n, 2))
+    X0 = np.zeros((n, 2))
+    it, X = p.cg(B, X0, 1e-8)
+
+    def mv(P):
+        return O.dmml(st, sz, "S", far, Us, Vs, near, Ds, P, 1.0, 0.0,
+                      np.zeros_like(P))
+    it_ref, X_ref = O.cg(mv, B, X0, 1e-8)
+    res = np.linalg.norm(B - mv(X), axis=0) / np.linalg.norm(B, axis=0)
+    diff = np.linalg.norm(X - X_ref) / np.linalg.norm(X_ref)
+    _say(f"  iterations {it} (oracle {it_ref}); residual {res.max():.2e}; "
+         f"solution diff {diff:.2e}")
+    assert it >= 0 and abs(it - it_ref) <= 2
+    assert res.max() <= 1e-7
+    assert diff <= 1e-6
+    p.free_matrix()
