@@ -35,16 +35,16 @@ int pipeline_colnorm(const TileDesc *d_tiles, int ntiles, int which,
 
 // Block Gram-Schmidt applied twice for a panel of jb columns at j0 (each
 // pass: projection against the previous columns, then CGS2 inside the
-// panel with deflation).  which: 0 = Q (width k), 1 = Q2 (width p).
-// Columns whose norm after projection is at most tau_first * ref (ref =
-// d_ref[t*kmax+j], or 1 when d_ref is null) in the first pass, or
-// tau_second in the second, are set to zero and flagged in d_keep.  A tile
-// whose whole panel was dropped in the first pass is marked done; *d_active
-// counts the tiles that still have panels to process.
+// panel with deflation).  which: 0 = Q (width k), 1 = Q2 (width p; the
+// panel is read from Bt and written to Q2).  Columns whose norm after
+// projection is at most tau_first times their original norm in the first
+// pass, or tau_second in the second, are set to zero and flagged in
+// d_keep.  A tile whose whole panel was dropped in the first pass is marked
+// done; *d_active counts the tiles that still have panels to process.
 int pipeline_panel(const TileDesc *d_tiles, int ntiles, int which,
-        double *ws, int j0, int jb, const double *d_ref, int kmax,
-        double tau_first, double tau_second, int *d_keep, int rows_max,
-        int *d_active, hipStream_t s);
+        double *ws, int j0, int jb, int kmax, double tau_first,
+        double tau_second, int *d_keep, int rows_max, int *d_active,
+        hipStream_t s);
 
 // *d_active = number of tiles with a nonzero width (k or p).
 int pipeline_count_active(const TileDesc *d_tiles, int ntiles, int which,
@@ -54,9 +54,8 @@ int pipeline_count_active(const TileDesc *d_tiles, int ntiles, int which,
 int pipeline_compact(TileDesc *d_tiles, int ntiles, double *ws,
         const int *d_keep, int kmax, hipStream_t s);
 
-// From column norms of Bt: |B|_F^2, truncation p and tail; copies
-// Bt[:, :p] into the Q2 buffer.
-// The tail bound is (eta*tol)^2 |B|_F^2 with each tile's own tol.
+// From column norms of Bt: |B|_F^2, truncation p and tail.  The tail
+// bound is (eta*tol)^2 |B|_F^2 with each tile's own tol.
 int pipeline_truncate(TileDesc *d_tiles, int ntiles, double *ws,
         const double *d_bnorm, int kmax, double eta, int nmax,
         hipStream_t s);

@@ -121,9 +121,9 @@ int qr_panel_width = 4;
 // MFMA (0); set by starsh_hip_set_option("proj_int8").
 int proj_int8 = 1;
 
-int bcgs2(TileDesc *d_tiles, int ntiles, int which, double *ws,
-        const double *d_ref, int kmax, int width_max, int rows_max,
-        int *d_keep, double tau_first, int *d_active, hipStream_t s)
+int bcgs2(TileDesc *d_tiles, int ntiles, int which, double *ws, int kmax,
+        int width_max, int rows_max, int *d_keep, double tau_first,
+        int *d_active, hipStream_t s)
 {
     const int pb = rows_max <= 2048 && qr_panel_width != 4 ? 8 : 4;
     TRY(probe.init());
@@ -138,7 +138,7 @@ int bcgs2(TileDesc *d_tiles, int ntiles, int which, double *ws,
             if(probe.h[slot] == 0)
                 break;
         }
-        TRY(pipeline_panel(d_tiles, ntiles, which, ws, j0, pb, d_ref, kmax,
+        TRY(pipeline_panel(d_tiles, ntiles, which, ws, j0, pb, kmax,
                     tau_first, TAU_SECOND, d_keep, rows_max, d_active, s));
         const int slot = panel % NSLOT;
         HIP_TRY(hipMemcpyAsync(probe.h+slot, d_active, sizeof(int),
@@ -411,12 +411,10 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     dump("omega", d_ws+h[0].om_off, (int64_t)h[0].n*h[0].k, s);
     dump("Y", d_ws+h[0].q_off, (int64_t)h[0].m*h[0].k, s);
     // 2. orthonormal basis with deflation, kept columns to the front
-    TRY(pipeline_colnorm(d_tiles, nt, 0, d_ws, (double *)norms.p, kmax, s));
     // columns of panels that are never launched count as dropped
     HIP_TRY(hipMemsetAsync(keep.p, 0, sizeof(int)*(size_t)nt*kmax, s));
-    TRY(bcgs2(d_tiles, nt, 0, d_ws, (double *)norms.p, kmax, kmax, mmax,
-                (int *)keep.p, in->tau > 0 ? in->tau : TAU_FIRST,
-                (int *)active.p, s));
+    TRY(bcgs2(d_tiles, nt, 0, d_ws, kmax, kmax, mmax, (int *)keep.p,
+                in->tau > 0 ? in->tau : TAU_FIRST, (int *)active.p, s));
     dump("Qfull", d_ws+h[0].q_off, (int64_t)h[0].m*h[0].k, s);
     TRY(pipeline_compact(d_tiles, nt, d_ws, (int *)keep.p, kmax, s));
     dump("Q", d_ws+h[0].q_off, (int64_t)h[0].m*h[0].k, s);
@@ -446,8 +444,8 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     // 5. Bt[:, :p] = Q2 R2
     if(pmax > 0)
     {
-        TRY(bcgs2(d_tiles, nt, 1, d_ws, (double *)norms.p, kmax, pmax, nmax,
-                    (int *)keep.p, TAU_FIRST, (int *)active.p, s));
+        TRY(bcgs2(d_tiles, nt, 1, d_ws, kmax, pmax, nmax, (int *)keep.p,
+                    TAU_FIRST, (int *)active.p, s));
         dump("Q2", d_ws+h[0].q2_off, (int64_t)h[0].n*h[0].p, s);
         TRY(pipeline_gemm(GM_R2, d_tiles, nt, pmax, pmax, g, s));
         dump("R2", d_ws+h[0].r_off, (int64_t)h[0].k*h[0].k, s);

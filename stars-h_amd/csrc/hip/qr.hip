@@ -109,17 +109,20 @@ __device__ inline int halving_index(int lane)
 //      read once, and its CH x PB dot products are reduced by
 //      halving_sum, then across the waves in LDS;
 //   2. CGS2 inside the panel with deflation: a column whose norm after
-//      projection is at most tau times its reference norm (ref in the
-//      first pass, or 1 when ref is null; 1 in the second pass, the column
-//      having been normalised) is set to zero and flagged in keep.
+//      projection is at most tau times its reference norm (in the first
+//      pass the norm of the original column, computed when the panel is
+//      loaded; 1 in the second pass, the column having been normalised) is
+//      set to zero and flagged in keep.
+// For Q2 the panel is read from Bt (Q2 starts as Bt[:, :p]) and written to
+// Q2.
 // The panel lives in registers: thread t holds rows t, t+NT, ... (RPT of
 // them).  A tile whose whole panel is dropped in the first pass is marked
 // done (its range is exhausted); *active counts the tiles that still have
 // panels to process.
 template <int NT, int RPT, int PB>
 __global__ __launch_bounds__(NT) void bcgs_kernel(TileDesc *tiles, int which,
-        double *ws, int j0, const double *ref, int kmax, double tau2_first,
-        double tau2_second, int *keep, int *active)
+        double *ws, int j0, int kmax, double tau2_first, double tau2_second,
+        int *keep, int *active)
 {
     constexpr int NW = NT/64, CH = 4, V = CH*PB;
     __shared__ double red[NW][V];
@@ -144,6 +147,8 @@ __global__ __launch_bounds__(NT) void bcgs_kernel(TileDesc *tiles, int which,
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double *Qb = ws+(q2 ? td.q2_off : td.q_off);
     double *W = Qb+(int64_t)j0*rows;
+    // Q2 starts as Bt[:, :p]: its panel is read from Bt and written to Q2
+    const double *Wsrc = q2 ? ws+td.bt_off+(int64_t)j0*rows : W;
     double w[RPT][PB];
 #pragma unroll
     for(int r = 0; r < RPT; r++)
@@ -151,7 +156,36 @@ __global__ __launch_bounds__(NT) void bcgs_kernel(TileDesc *tiles, int which,
         const int i = threadIdx.x+r*NT;
 #pragma unroll
         for(int c = 0; c < PB; c++)
-            w[r][c] = (i < rows && c < jb) ? W[(int64_t)c*rows+i] : 0.0;
+            w[r][c] = (i < rows && c < jb) ? Wsrc[(int64_t)c*rows+i] : 0.0;
+    }
+    // squared norms of the panel's original columns (the deflation
+    // reference of the first pass)
+    double ref2[PB];
+    {
+        double v[PB];
+#pragma unroll
+        for(int c = 0; c < PB; c++)
+        {
+            double d = 0.0;
+#pragma unroll
+            for(int r = 0; r < RPT; r++)
+                d += w[r][c]*w[r][c];
+            v[c] = d;
+        }
+        const double tot = halving_sum<PB>(v, lane);
+        if((lane & ((64/PB)-1)) == 0)
+            red[wave][halving_index<PB>(lane)] = tot;
+        __syncthreads();
+#pragma unroll
+        for(int c = 0; c < PB; c++)
+        {
+            double sum = 0.0;
+#pragma unroll
+            for(int q2w = 0; q2w < NW; q2w++)
+                sum += red[q2w][c];
+            ref2[c] = sum;
+        }
+        __syncthreads();
     }
     const int my_idx = halving_index<V>(lane);
     const bool writer = (lane & ((64/V)-1)) == 0;
@@ -260,8 +294,7 @@ __global__ __launch_bounds__(NT) void bcgs_kernel(TileDesc *tiles, int which,
                 for(int q2w = 0; q2w < NW; q2w++)
                     n2 += pred[rb][q2w][0];
                 rb ^= 1;
-                const double r2 = pass == 0 && ref ?
-                    ref[(int64_t)blockIdx.x*kmax+j0+c] : 1.0;
+                const double r2 = pass == 0 ? ref2[c] : 1.0;
                 const double tau2 = pass == 0 ? tau2_first : tau2_second;
                 const bool drop = !(n2 > tau2*r2) || n2 == 0.0;
                 const double inv = drop ? 0.0 : 1.0/sqrt(n2);
@@ -350,44 +383,35 @@ __global__ __launch_bounds__(256) void compact_kernel(TileDesc *tiles,
 }
 
 // |B|_F^2, truncation point p (smallest p >= 1 whose row tail of B is at
-// most (eta*tol)^2 |B|_F^2) and the tail; copies Bt[:, :p] into Q2.
+// most (eta*tol)^2 |B|_F^2) and the tail.
 __global__ __launch_bounds__(256) void truncate_kernel(TileDesc *tiles,
-        double *ws, const double *bnorm2, int kmax, double eta)
+        int ntiles, const double *bnorm2, int kmax, double eta)
 {
-    TileDesc &td = tiles[blockIdx.x];
-    const double *nb = bnorm2+(int64_t)blockIdx.x*kmax;
-    __shared__ int sp;
-    if(threadIdx.x == 0)
+    const int t = blockIdx.x*blockDim.x+threadIdx.x;
+    if(t >= ntiles)
+        return;
+    TileDesc &td = tiles[t];
+    const double *nb = bnorm2+(int64_t)t*kmax;
+    double total = 0.0;
+    for(int c = 0; c < td.kc; c++)
+        total += nb[c];
+    const double eta_tol = eta*td.tol;
+    double limit = eta_tol*eta_tol*total;
+    int p = td.kc;
+    double tail = 0.0;
+    while(p > 1 && tail+nb[p-1] <= limit)
     {
-        double total = 0.0;
-        for(int c = 0; c < td.kc; c++)
-            total += nb[c];
-        const double eta_tol = eta*td.tol;
-        double limit = eta_tol*eta_tol*total;
-        int p = td.kc;
-        double tail = 0.0;
-        while(p > 1 && tail+nb[p-1] <= limit)
-        {
-            tail += nb[p-1];
-            p--;
-        }
-        if(total == 0.0)
-        {
-            p = 0;
-            tail = 0.0;
-        }
-        td.p = p;
-        td.tail2 = tail;
-        td.bnorm2 = total;
-        sp = p;
+        tail += nb[p-1];
+        p--;
     }
-    __syncthreads();
-    const int p = sp;
-    const int64_t cnt = (int64_t)td.n*p;
-    const double *src = ws+td.bt_off;
-    double *dst = ws+td.q2_off;
-    for(int64_t e = threadIdx.x; e < cnt; e += blockDim.x)
-        dst[e] = src[e];
+    if(total == 0.0)
+    {
+        p = 0;
+        tail = 0.0;
+    }
+    td.p = p;
+    td.tail2 = tail;
+    td.bnorm2 = total;
 }
 
 // Number of tiles that take part in a BCGS2 run (width > 0).
@@ -438,9 +462,9 @@ int pipeline_count_active(const TileDesc *d_tiles, int ntiles, int which,
 }
 
 int pipeline_panel(const TileDesc *d_tiles, int ntiles, int which,
-        double *ws, int j0, int jb, const double *d_ref, int kmax,
-        double tau_first, double tau_second, int *d_keep, int rows_max,
-        int *d_active, hipStream_t s)
+        double *ws, int j0, int jb, int kmax, double tau_first,
+        double tau_second, int *d_keep, int rows_max, int *d_active,
+        hipStream_t s)
 {
     if(ntiles <= 0)
         return STARSH_SUCCESS;
@@ -448,8 +472,8 @@ int pipeline_panel(const TileDesc *d_tiles, int ntiles, int which,
     const double tau2a = tau_first*tau_first, tau2b = tau_second*tau_second;
 #define BCGS_LAUNCH(NT, RPT, PB) \
         hipLaunchKernelGGL((bcgs_kernel<NT, RPT, PB>), dim3(ntiles), \
-                dim3(NT), 0, s, tiles, which, ws, j0, d_ref, kmax, tau2a, \
-                tau2b, d_keep, d_active)
+                dim3(NT), 0, s, tiles, which, ws, j0, kmax, tau2a, tau2b, \
+                d_keep, d_active)
     if(jb == 4 && rows_max <= 1024)
         BCGS_LAUNCH(256, 4, 4);
     else if(jb == 4 && rows_max <= 2048)
@@ -485,8 +509,9 @@ int pipeline_truncate(TileDesc *d_tiles, int ntiles, double *ws,
     (void)nmax;
     if(ntiles <= 0)
         return STARSH_SUCCESS;
-    hipLaunchKernelGGL(truncate_kernel, dim3(ntiles), dim3(256), 0, s,
-            d_tiles, ws, d_bnorm, kmax, eta);
+    (void)ws;
+    hipLaunchKernelGGL(truncate_kernel, dim3((ntiles+255)/256), dim3(256), 0,
+            s, d_tiles, ntiles, d_bnorm, kmax, eta);
     HIP_TRY(hipGetLastError());
     return STARSH_SUCCESS;
 }
