@@ -86,15 +86,29 @@ template <> struct ModeTraits<GM_R2> { static constexpr int a = 1; };
 template <> struct ModeTraits<GM_FORM_U> { static constexpr int a = 0; };
 template <> struct ModeTraits<GM_FORM_V> { static constexpr int a = 0; };
 
+// grid.z > 1 splits K: block z covers K steps [z*ks, (z+1)*ks) and writes
+// its partial product to part + (tile*grid.z + z)*pld*pld (ld pld); the
+// partials are summed by ksum_kernel.
 template <int MODE>
 __global__ __launch_bounds__(256) void gemm_kernel(const TileDesc *tiles,
-        GemmArgs g)
+        GemmArgs g, double *part, int pld)
 {
     constexpr int AL = ModeTraits<MODE>::a;
     const TileDesc t = tiles[blockIdx.y];
-    const Op o = setup<MODE>(t, g);
+    Op o = setup<MODE>(t, g);
     if(!o.valid || o.M <= 0 || o.N <= 0)
         return;
+    int kbeg = 0;
+    if(gridDim.z > 1)
+    {
+        const int ks = ((o.K+gridDim.z-1)/gridDim.z+BK-1)/BK*BK;
+        kbeg = blockIdx.z*ks;
+        const int kend = kbeg+ks < o.K ? kbeg+ks : o.K;
+        o.C = part+((int64_t)blockIdx.y*gridDim.z+blockIdx.z)*pld*pld;
+        o.ldc = pld;
+        o.beta = 0.0;
+        o.K = kend;
+    }
     const int nbm = (o.M+BM-1)/BM;
     const int bm = blockIdx.x % nbm, bn = blockIdx.x / nbm;
     if(bn*BN >= o.N)
@@ -113,7 +127,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(const TileDesc *tiles,
         for(int b = 0; b < 2; b++)
             acc[a][b] = double4_t{0.0, 0.0, 0.0, 0.0};
 
-    for(int k0 = 0; k0 < o.K; k0 += BK)
+    for(int k0 = kbeg; k0 < o.K; k0 += BK)
     {
         // ---- stage A (BM x BK) as As[k][i]
         if(AL == 0)
@@ -198,20 +212,68 @@ __global__ __launch_bounds__(256) void gemm_kernel(const TileDesc *tiles,
         }
 }
 
+// R2[:p, :p] of every tile = sum of its nz partials (in order).
+__global__ __launch_bounds__(256) void ksum_kernel(const TileDesc *tiles,
+        const double *part, int pld, int nz, double *ws)
+{
+    const TileDesc t = tiles[blockIdx.y];
+    const int p = t.p;
+    const int64_t e = (int64_t)blockIdx.x*256+threadIdx.x;
+    if(p <= 0 || e >= (int64_t)p*p)
+        return;
+    const int i = (int)(e % p), j = (int)(e / p);
+    const double *src = part+(int64_t)blockIdx.y*nz*pld*pld+(int64_t)j*pld+i;
+    double sum = 0.0;
+    for(int z = 0; z < nz; z++)
+        sum += src[(int64_t)z*pld*pld];
+    ws[t.r_off+(int64_t)j*t.k+i] = sum;
+}
+
 template <int MODE>
 int launch(const TileDesc *d_tiles, int ntiles, int mmax, int nmax,
-        const GemmArgs &g, hipStream_t s)
+        const GemmArgs &g, hipStream_t s, int nz = 1, double *part = nullptr,
+        int pld = 0)
 {
     int nbm = (mmax+BM-1)/BM, nbn = (nmax+BN-1)/BN;
     if(nbm <= 0 || nbn <= 0 || ntiles <= 0)
         return STARSH_SUCCESS;
-    hipLaunchKernelGGL((gemm_kernel<MODE>), dim3(nbm*nbn, ntiles),
-            dim3(256), 0, s, d_tiles, g);
+    hipLaunchKernelGGL((gemm_kernel<MODE>), dim3(nbm*nbn, ntiles, nz),
+            dim3(256), 0, s, d_tiles, g, part, pld);
     HIP_TRY(hipGetLastError());
     return STARSH_SUCCESS;
 }
 
 } // namespace
+
+int pipeline_r2_split(int ntiles, int pmax, int kmax_rows)
+{
+    // at least 128 rows of K per block, at most 8 blocks, and the partials
+    // within 1 GiB
+    int nz = kmax_rows/128;
+    nz = nz < 1 ? 1 : (nz > 8 ? 8 : nz);
+    const double per = 8.0*ntiles*(double)pmax*pmax;
+    while(nz > 1 && per*nz > (double)(1 << 30))
+        nz--;
+    return nz;
+}
+
+int pipeline_r2(const TileDesc *d_tiles, int ntiles, int pmax, int nz,
+        double *d_part, const GemmArgs &g, hipStream_t s)
+{
+    if(ntiles <= 0 || pmax <= 0)
+        return STARSH_SUCCESS;
+    if(nz <= 1 || d_part == nullptr)
+        return launch<GM_R2>(d_tiles, ntiles, pmax, pmax, g, s);
+    const int info = launch<GM_R2>(d_tiles, ntiles, pmax, pmax, g, s, nz,
+            d_part, pmax);
+    if(info != STARSH_SUCCESS)
+        return info;
+    const int64_t cnt = (int64_t)pmax*pmax;
+    hipLaunchKernelGGL(ksum_kernel, dim3((unsigned)((cnt+255)/256), ntiles),
+            dim3(256), 0, s, d_tiles, d_part, pmax, nz, g.ws);
+    HIP_TRY(hipGetLastError());
+    return STARSH_SUCCESS;
+}
 
 int pipeline_gemm(int mode, const TileDesc *d_tiles, int ntiles, int mmax,
         int nmax, const GemmArgs &g, hipStream_t s)

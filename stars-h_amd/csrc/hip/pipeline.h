@@ -23,6 +23,13 @@ struct GemmArgs
 int pipeline_gemm(int mode, const TileDesc *d_tiles, int ntiles, int mmax,
         int nmax, const GemmArgs &g, hipStream_t s);
 
+// R2 = Q2^T Bt[:, :p] with K (the n rows) split over nz workgroups per
+// output block; d_part holds ntiles * nz * pmax^2 doubles of partial
+// products, summed in order.  nz = 1 (or no d_part) runs unsplit.
+int pipeline_r2_split(int ntiles, int pmax, int kmax_rows);
+int pipeline_r2(const TileDesc *d_tiles, int ntiles, int pmax, int nz,
+        double *d_part, const GemmArgs &g, hipStream_t s);
+
 // Sketch (proj = 0) or projection (proj = 1) with the kernel tile generated
 // in registers (gen_gemm.hip).
 int pipeline_gen_gemm(int proj, const TileDesc *d_tiles, int ntiles,

@@ -354,7 +354,8 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     if(in->rows_out)
         for(int t = 0; t < nt; t++)
             in->rows_out[t] = h[t].k;
-    DeviceBuf tiles, keep, norms, om8, active, qd;
+    // part: split-K partials of R2, kept until the batch has synchronised
+    DeviceBuf tiles, keep, norms, om8, active, qd, part;
     TRY(ws_pool.reserve(sizeof(double)*off));
     if(in->sketch_int8)
         TRY(om8.alloc(off8));
@@ -447,7 +448,10 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
         TRY(bcgs2(d_tiles, nt, 1, d_ws, kmax, pmax, nmax, (int *)keep.p,
                     TAU_FIRST, (int *)active.p, s));
         dump("Q2", d_ws+h[0].q2_off, (int64_t)h[0].n*h[0].p, s);
-        TRY(pipeline_gemm(GM_R2, d_tiles, nt, pmax, pmax, g, s));
+        const int nz = pipeline_r2_split(nt, pmax, nmax);
+        if(nz > 1)
+            TRY(part.alloc(sizeof(double)*(size_t)nt*nz*pmax*pmax));
+        TRY(pipeline_r2(d_tiles, nt, pmax, nz, (double *)part.p, g, s));
         dump("R2", d_ws+h[0].r_off, (int64_t)h[0].k*h[0].k, s);
     }
     // 6. core SVD and rank
