@@ -41,6 +41,8 @@ __global__ __launch_bounds__(256) void tlr_coef_kernel(const tlr_tile *tiles,
         {
             const double *xv = x+(int64_t)q*ldx+x0;
             double s[4] = {0.0, 0.0, 0.0, 0.0};
+            // four rows per lane in flight at a time
+#pragma unroll 4
             for(int l = lane; l < len; l += 64)
             {
                 const double xl = xv[l];
@@ -88,6 +90,21 @@ __global__ __launch_bounds__(256) void tlr_rows_kernel(const tlr_chunk *chunks,
                 const double *T = tmp+en.src_off*nrhs;
                 double a4[4][RHS_GROUP] = {};
                 int c = 0;
+                // eight factor columns in flight at a time
+                for(; c+8 <= en.cols; c += 8)
+                {
+                    double f8[8];
+#pragma unroll
+                    for(int u = 0; u < 8; u++)
+                        f8[u] = F[(int64_t)(c+u)*en.ld];
+#pragma unroll
+                    for(int u = 0; u < 8; u++)
+#pragma unroll
+                        for(int q = 0; q < RHS_GROUP; q++)
+                            if(q < nq)
+                                a4[u & 3][q] = fma(f8[u], T[c+u+
+                                        (int64_t)(q0+q)*en.cols], a4[u & 3][q]);
+                }
                 for(; c+4 <= en.cols; c += 4)
 #pragma unroll
                     for(int u = 0; u < 4; u++)
