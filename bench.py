@@ -154,6 +154,7 @@ def run_form(S, w, backend, steps, warmup, autotune):
     t = sum(dev) / len(dev)
     res = {
         "device_seconds": t,
+        "step_device_seconds": [round(x, 6) for x in dev],
         "api_wall_seconds": sum(wall) / len(wall),
         # summed over the launches of one step
         "dominant_kernel_seconds": sum(dom) / len(dom),
@@ -202,8 +203,8 @@ def cpu_baseline(w, p, nsample):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="C2", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-sample", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")

@@ -24,7 +24,10 @@ static const int bucket_widths[] = {32, 64, 128, 256, 512};
 #define NBUCKETS ((int)(sizeof(bucket_widths)/sizeof(bucket_widths[0])))
 
 /* First bucket forced by starsh_hip_set_option("mixed_first_bucket"), or -1
- * for the schedule table (then 0 when the problem has no entry). */
+ * for the schedule table, then DEFAULT_FIRST (width 64) when the problem has
+ * no entry: on every workload measured (2D sqexp and exp, 3D Matern,
+ * Coulomb and sin(kr)/r) width 64 beat width 32 as the first bucket. */
+#define DEFAULT_FIRST 1
 static int forced_bucket = -1;
 /* Tiles still pending after the first bucket of the last call. */
 static long long last_pending_after_first = 0;
@@ -221,8 +224,9 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
     double tau = 0.1*tmin;
     tau = tau > 1e-8 ? 1e-8 : tau;
     tau = tau < 1e-13 ? 1e-13 : tau;
-    /* Four 7-bit digits represent A to 2^-28 of a chunk's largest entry;
-     * tighter tolerances keep the FP64 sketch (still rank-bucketed). */
+    /* Four byte digits represent A to 2^-31 of a chunk's largest entry;
+     * tighter tolerances keep the FP64 sketch and projection (still
+     * rank-bucketed). */
     const int int8 = tmin >= 3e-8;
     if(!int8)
         tau = 1e-13;
@@ -235,7 +239,7 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
             sched_key(B->hp+q, B->F[q], B->maxrank[q], B->tol[q], &key);
             first[q] = sched_first(&key);
             if(first[q] < 0)
-                first[q] = 0;
+                first[q] = DEFAULT_FIRST;
         }
     }
     STARSH_int npend = info == STARSH_SUCCESS ? ntiles : 0;
@@ -321,21 +325,93 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
  * run), keeps the fastest by device time, records it in the schedule table
  * (and in the STARSH_HIP_SCHEDULE file) and returns it in *first_out.  The
  * candidates stop at the first bucket that finishes every tile. */
+/* The tile lists of a format, kept so that every trial of the autotuner
+ * starts from the format the caller passed (a compression rewrites it when
+ * it finds false far-field tiles). */
+typedef struct
+{
+    STARSH_int nfar, nnear;
+    STARSH_int *far, *near;
+} format_lists;
+
+static int lists_save(const STARSH_blrf *F, format_lists *L)
+{
+    L->nfar = F->nblocks_far;
+    L->nnear = F->nblocks_near;
+    L->far = malloc(sizeof(STARSH_int)*2*(L->nfar > 0 ? L->nfar : 1));
+    L->near = malloc(sizeof(STARSH_int)*2*(L->nnear > 0 ? L->nnear : 1));
+    if(!L->far || !L->near)
+        return STARSH_MALLOC_ERROR;
+    if(L->nfar > 0)
+        memcpy(L->far, F->block_far, sizeof(STARSH_int)*2*L->nfar);
+    if(L->nnear > 0)
+        memcpy(L->near, F->block_near, sizeof(STARSH_int)*2*L->nnear);
+    return STARSH_SUCCESS;
+}
+
+/* Rebuilds *F from the saved lists if a trial changed them. */
+static int lists_restore(STARSH_blrf *F, const format_lists *L)
+{
+    if(F->nblocks_far == L->nfar && F->nblocks_near == L->nnear)
+        return STARSH_SUCCESS;
+    STARSH_int *far = malloc(sizeof(STARSH_int)*2*(L->nfar > 0 ? L->nfar :
+                1));
+    STARSH_int *near = malloc(sizeof(STARSH_int)*2*(L->nnear > 0 ?
+                L->nnear : 1));
+    if(!far || !near)
+    {
+        free(far);
+        free(near);
+        return STARSH_MALLOC_ERROR;
+    }
+    memcpy(far, L->far, sizeof(STARSH_int)*2*L->nfar);
+    memcpy(near, L->near, sizeof(STARSH_int)*2*L->nnear);
+    STARSH_blrf *F2;
+    int info = starsh_blrf_new_from_coo(&F2, F->problem, F->symm,
+            F->row_cluster, F->col_cluster, L->nfar, far, L->nnear, near,
+            F->type);
+    if(info != STARSH_SUCCESS)
+        return info;
+    if(L->nfar == 0)
+        free(far);
+    if(L->nnear == 0)
+        free(near);
+    STARSH_blrf tmp = *F;
+    *F = *F2;
+    *F2 = tmp;
+    starsh_blrf_free(F2);
+    return STARSH_SUCCESS;
+}
+
 int starsh_hip_autotune(STARSH_blrf *format, int maxrank, double tol,
         int onfly, int *first_out, double *seconds_out)
 {
     if(format == NULL)
         return STARSH_WRONG_PARAMETER;
+    format_lists lists = {0, 0, NULL, NULL};
+    int info = lists_save(format, &lists);
+    if(info != STARSH_SUCCESS)
+    {
+        free(lists.far);
+        free(lists.near);
+        return info;
+    }
     const int saved = forced_bucket;
     STARSH_blrm *M = NULL;
     forced_bucket = 0;
-    int info = starsh_blrm__drsdd_hip_mixed(&M, format, maxrank, tol, onfly);
+    info = starsh_blrm__drsdd_hip_mixed(&M, format, maxrank, tol, onfly);
+    if(info == STARSH_SUCCESS)
+    {
+        starsh_blrm_free(M);
+        info = lists_restore(format, &lists);
+    }
     if(info != STARSH_SUCCESS)
     {
         forced_bucket = saved;
+        free(lists.far);
+        free(lists.near);
         return info;
     }
-    starsh_blrm_free(M);
     int best = 0;
     double best_t = 1e300;
     for(int b = 0; b < NBUCKETS; b++)
@@ -348,6 +424,9 @@ int starsh_hip_autotune(STARSH_blrf *format, int maxrank, double tol,
         starsh_hip_stats st;
         starsh_hip_get_stats(&st);
         starsh_blrm_free(M);
+        info = lists_restore(format, &lists);
+        if(info != STARSH_SUCCESS)
+            break;
         if(seconds_out)
             seconds_out[b] = st.device_seconds;
         if(st.device_seconds < best_t)
@@ -359,6 +438,8 @@ int starsh_hip_autotune(STARSH_blrf *format, int maxrank, double tol,
             break;
     }
     forced_bucket = saved;
+    free(lists.far);
+    free(lists.near);
     if(info != STARSH_SUCCESS)
         return info;
     hip_problem hp;
