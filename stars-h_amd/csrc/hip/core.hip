@@ -61,6 +61,57 @@ __device__ inline bool rotate_pair(double *wa, double *wb, int p, int g,
     return true;
 }
 
+// Columns of up to CREG*64 rows held in registers by one wave (lane l holds
+// rows l, l+64, ...).
+constexpr int CREG = 16;
+
+// rotate_pair with column a in the wave's registers (ca) and column b in
+// memory: the same sums in the same order, so the same rotation; b is read
+// once and written once.
+__device__ inline bool rotate_reg(double (&ca)[CREG], double *wb, int p,
+        int lane, double ctol)
+{
+    double cb[CREG];
+    double al = 0.0, be = 0.0, ga = 0.0;
+#pragma unroll
+    for(int t = 0; t < CREG; t++)
+    {
+        const int i = lane+64*t;
+        cb[t] = i < p ? wb[i] : 0.0;
+        if(i < p)
+        {
+            double u = ca[t], v = cb[t];
+            al += u*u;
+            be += v*v;
+            ga += u*v;
+        }
+    }
+    al = group_sum(al, 64);
+    be = group_sum(be, 64);
+    ga = group_sum(ga, 64);
+    if(!(al > 0.0 && be > 0.0 && fabs(ga) > ctol*sqrt(al*be)))
+        return false;
+    double zeta = (be-al)/(2.0*ga);
+    double t;
+    if(fabs(zeta) > 1e150)
+        t = 0.5/zeta;
+    else
+        t = (zeta >= 0.0 ? 1.0 : -1.0)/(fabs(zeta)+sqrt(1.0+zeta*zeta));
+    double c = 1.0/sqrt(1.0+t*t), s = c*t;
+#pragma unroll
+    for(int q = 0; q < CREG; q++)
+    {
+        const int i = lane+64*q;
+        if(i < p)
+        {
+            double u = ca[q], v = cb[q];
+            ca[q] = c*u-s*v;
+            wb[i] = s*u+c*v;
+        }
+    }
+    return true;
+}
+
 // W: p x p working matrix (ld = ldw), in LDS or global memory.  A group
 // of g threads handles one column pair, g consecutive rows at a time.  In
 // LDS the groups are as small as needed to give every pair of a round its
@@ -175,8 +226,18 @@ __device__ void jacobi_blocked(double *W, int p, double *L, int bw,
                 }
                 __syncthreads();
             }
-            // cross pairs with every later block
+            // cross pairs with every later block: wave a keeps column a of
+            // block I in its registers for all of them (bw <= waves)
             double *LJ = L+(int64_t)bw*p;
+            const int a = wave;
+            const bool own = a < wi;
+            double ca[CREG];
+#pragma unroll
+            for(int t = 0; t < CREG; t++)
+            {
+                const int i = lane+64*t;
+                ca[t] = own && i < p ? L[(int64_t)a*p+i] : 0.0;
+            }
             for(int J = I+1; J < nb; J++)
             {
                 const int wj = width(J);
@@ -184,19 +245,24 @@ __device__ void jacobi_blocked(double *W, int p, double *L, int bw,
                 __syncthreads();
                 for(int round = 0; round < bw; round++)
                 {
-                    for(int a = wave; a < wi; a += nwaves)
-                    {
-                        const int b = (a+round) % bw;
-                        if(b < wj && rotate_pair(L+(int64_t)a*p,
-                                    LJ+(int64_t)b*p, p, 64, lane, ctol) &&
-                                lane == 0)
-                            atomicAdd(&rotations, 1);
-                    }
+                    const int b = (a+round) % bw;
+                    if(own && b < wj && rotate_reg(ca, LJ+(int64_t)b*p, p,
+                                lane, ctol) && lane == 0)
+                        atomicAdd(&rotations, 1);
                     __syncthreads();
                 }
                 store(J, LJ);
                 __syncthreads();
             }
+            if(own)
+#pragma unroll
+                for(int t = 0; t < CREG; t++)
+                {
+                    const int i = lane+64*t;
+                    if(i < p)
+                        L[(int64_t)a*p+i] = ca[t];
+                }
+            __syncthreads();
             store(I, L);
             __syncthreads();
         }
@@ -256,9 +322,14 @@ __global__ __launch_bounds__(NT) void core_kernel(TileDesc *tiles,
         W[(int64_t)j*ldw+i] = R2[(int64_t)i*k+j];
     }
     __syncthreads();
-    int sweeps;
+    int sweeps = 0;
     if(big)
-        jacobi_blocked(W, p, lds, bw, &sweeps);
+    {
+        // the blocked sweep gives each wave one column of a block, so it
+        // runs only in the 16-wave kernel (launched with bw <= 16)
+        if constexpr(NT == CORE_THREADS_BIG)
+            jacobi_blocked(W, p, lds, bw, &sweeps);
+    }
     else
         jacobi<NT>(W, ldw, p, true, &sweeps);
     __syncthreads();
