@@ -273,9 +273,162 @@ __device__ void jacobi_blocked(double *W, int p, double *L, int bw,
     *sweeps_out = sweep;
 }
 
+// Column pairs handled by half-waves (32 lanes) for p <= CR32*32.
+constexpr int CR32 = 17;
+
+// rotate_reg for a group of 32 lanes (gl = lane in the group; rows gl,
+// gl+32, ...).
+__device__ inline bool rotate_reg32(double (&ca)[CR32], double *wb, int p,
+        int gl, double ctol)
+{
+    double cb[CR32];
+    double al = 0.0, be = 0.0, ga = 0.0;
+#pragma unroll
+    for(int t = 0; t < CR32; t++)
+    {
+        const int i = gl+32*t;
+        cb[t] = i < p ? wb[i] : 0.0;
+        if(i < p)
+        {
+            double u = ca[t], v = cb[t];
+            al += u*u;
+            be += v*v;
+            ga += u*v;
+        }
+    }
+    al = group_sum(al, 32);
+    be = group_sum(be, 32);
+    ga = group_sum(ga, 32);
+    if(!(al > 0.0 && be > 0.0 && fabs(ga) > ctol*sqrt(al*be)))
+        return false;
+    double zeta = (be-al)/(2.0*ga);
+    double t;
+    if(fabs(zeta) > 1e150)
+        t = 0.5/zeta;
+    else
+        t = (zeta >= 0.0 ? 1.0 : -1.0)/(fabs(zeta)+sqrt(1.0+zeta*zeta));
+    double c = 1.0/sqrt(1.0+t*t), s = c*t;
+#pragma unroll
+    for(int q = 0; q < CR32; q++)
+    {
+        const int i = gl+32*q;
+        if(i < p)
+        {
+            double u = ca[q], v = cb[q];
+            ca[q] = c*u-s*v;
+            wb[i] = s*u+c*v;
+        }
+    }
+    return true;
+}
+
+// jacobi_blocked with blocks of 32 columns, one column pair per half-wave
+// (32 pairs at a time in 1024 threads), for p <= 544.  Only one block sits
+// in LDS (L: 32 p doubles): block I is rotated there within itself, then
+// half-wave a takes its column a into registers, and the later blocks J
+// pass through L one after the other; column a goes back to W at the end.
+// The same pairs in the same order as jacobi_blocked with bw = 32.
+__device__ void jacobi_blocked32(double *W, int p, double *L, int *sweeps_out)
+{
+    constexpr int BW = 32;
+    __shared__ int rotations;
+    const int nb = (p+BW-1)/BW;
+    const int hg = threadIdx.x >> 5, gl = threadIdx.x & 31;
+    const int ngroups = blockDim.x >> 5;
+    const double ctol = (double)p*DBL_EPSILON;
+    auto width = [&](int blk) { return min(BW, p-blk*BW); };
+    auto load = [&](int blk) {
+        const int cnt = width(blk)*p;
+        const double *src = W+(int64_t)blk*BW*p;
+        for(int e = threadIdx.x; e < cnt; e += blockDim.x)
+            L[e] = src[e];
+    };
+    auto store = [&](int blk) {
+        const int cnt = width(blk)*p;
+        double *dst = W+(int64_t)blk*BW*p;
+        for(int e = threadIdx.x; e < cnt; e += blockDim.x)
+            dst[e] = L[e];
+    };
+    int sweep;
+    for(sweep = 0; sweep < 60; sweep++)
+    {
+        if(threadIdx.x == 0)
+            rotations = 0;
+        __syncthreads();
+        for(int I = 0; I < nb; I++)
+        {
+            const int wi = width(I);
+            load(I);
+            __syncthreads();
+            // pairs inside block I (circle method)
+            const int we = wi+(wi & 1);
+            for(int round = 0; round < we-1; round++)
+            {
+                for(int pi = hg; pi < we/2; pi += ngroups)
+                {
+                    int x = pi, y = we-1-pi;
+                    int a = x == 0 ? 0 : 1+(x-1+round) % (we-1);
+                    int b = 1+(y-1+round) % (we-1);
+                    if(a > b)
+                    {
+                        int tmp = a;
+                        a = b;
+                        b = tmp;
+                    }
+                    if(b < wi && rotate_pair(L+(int64_t)a*p, L+(int64_t)b*p,
+                                p, 32, gl, ctol) && gl == 0)
+                        atomicAdd(&rotations, 1);
+                }
+                __syncthreads();
+            }
+            // cross pairs: half-wave a keeps column a of block I
+            const int a = hg;
+            const bool own = a < wi;
+            double ca[CR32];
+#pragma unroll
+            for(int t = 0; t < CR32; t++)
+            {
+                const int i = gl+32*t;
+                ca[t] = own && i < p ? L[(int64_t)a*p+i] : 0.0;
+            }
+            __syncthreads();
+            for(int J = I+1; J < nb; J++)
+            {
+                const int wj = width(J);
+                load(J);
+                __syncthreads();
+                for(int round = 0; round < BW; round++)
+                {
+                    const int b = (a+round) % BW;
+                    if(own && b < wj && rotate_reg32(ca, L+(int64_t)b*p, p,
+                                gl, ctol) && gl == 0)
+                        atomicAdd(&rotations, 1);
+                    __syncthreads();
+                }
+                store(J);
+                __syncthreads();
+            }
+            if(own)
+#pragma unroll
+                for(int t = 0; t < CR32; t++)
+                {
+                    const int i = gl+32*t;
+                    if(i < p)
+                        W[((int64_t)I*BW+a)*p+i] = ca[t];
+                }
+            __syncthreads();
+        }
+        if(rotations == 0)
+            break;
+        __syncthreads();
+    }
+    *sweeps_out = sweep;
+}
+
 // Dynamic LDS: sig[pmax] doubles, then the Jacobi buffer (W, lds_p^2
 // doubles, for the small tiles; two column blocks, 2 bw pmax doubles, for
-// the big ones), then perm[pmax] ints.  With big == 0 the kernel takes the
+// the big ones, or one block of 32 columns with bw = 32), then perm[pmax]
+// ints.  With big == 0 the kernel takes the
 // tiles with p <= lds_p (W in LDS); with big != 0 those with p > LDS_PMAX
 // (W in their workspace, blocked Jacobi through LDS).
 template <int NT>
@@ -285,7 +438,8 @@ __global__ __launch_bounds__(NT) void core_kernel(TileDesc *tiles,
     extern __shared__ double dyn[];
     double *sig = dyn;
     double *lds = dyn+pmax;
-    int *perm = (int *)(lds+(big ? (int64_t)2*bw*pmax :
+    // bw = 32: one block in LDS (jacobi_blocked32); else two
+    int *perm = (int *)(lds+(big ? (int64_t)(bw == 32 ? 1 : 2)*bw*pmax :
                 (int64_t)lds_p*lds_p));
     __shared__ int s_rank;
     TileDesc &td = tiles[blockIdx.x];
@@ -325,10 +479,16 @@ __global__ __launch_bounds__(NT) void core_kernel(TileDesc *tiles,
     int sweeps = 0;
     if(big)
     {
-        // the blocked sweep gives each wave one column of a block, so it
-        // runs only in the 16-wave kernel (launched with bw <= 16)
+        // the blocked sweeps give each wave (bw <= 16) or half-wave
+        // (bw = 32) one column of a block, so they run only in the 16-wave
+        // kernel
         if constexpr(NT == CORE_THREADS_BIG)
-            jacobi_blocked(W, p, lds, bw, &sweeps);
+        {
+            if(bw == 32)
+                jacobi_blocked32(W, p, lds, &sweeps);
+            else
+                jacobi_blocked(W, p, lds, bw, &sweeps);
+        }
     }
     else
         jacobi<NT>(W, ldw, p, true, &sweeps);
@@ -426,11 +586,12 @@ int pipeline_core(TileDesc *d_tiles, int ntiles, double *ws, int pmax,
     HIP_TRY(hipGetLastError());
     if(pm > LDS_PMAX)
     {
-        // two column blocks of bw columns in LDS: 16 columns up to 576
-        // rows, 8 beyond (at most 160 KiB with sig and perm)
-        const int bw = pm <= 576 ? 16 : 8;
-        const size_t bb = sizeof(double)*((size_t)pm+(size_t)2*bw*pm)+
-            sizeof(int)*(size_t)pm;
+        // one block of 32 columns in LDS up to 32*CR32 rows (half-waves
+        // hold their column in registers), else two blocks of 16 columns
+        // up to 576 rows, 8 beyond (at most 160 KiB with sig and perm)
+        const int bw = pm <= 32*CR32 ? 32 : (pm <= 576 ? 16 : 8);
+        const size_t bb = sizeof(double)*((size_t)pm+
+                (size_t)(bw == 32 ? 1 : 2)*bw*pm)+sizeof(int)*(size_t)pm;
         HIP_TRY(hipFuncSetAttribute(
                     (const void *)core_kernel<CORE_THREADS_BIG>,
                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bb));
