@@ -22,9 +22,7 @@ constexpr int LDS_PMAX = 128;
 
 __device__ inline double group_sum(double v, int g)
 {
-    for(int o = g >> 1; o > 0; o >>= 1)
-        v += __shfl_xor(v, o, 64);
-    return v;
+    return group_sum_lanes(v, g);
 }
 
 // One-sided Jacobi step on the column pair (wa, wb) of length p by a group

@@ -303,6 +303,95 @@ struct PointSets
     int64_t nc;
 };
 
+// Cross-lane exchanges on the vector ALU, without the LDS crossbar of
+// ds_bpermute.  At level O an aligned group of 2 O lanes is split into two
+// halves of O lanes, and the partner of a lane lies in the other half:
+//   O = 32, 16: lane ^ O (v_permlane32_swap, v_permlane16_swap);
+//   O = 8: lane 15 - l within its row of 16 (DPP row_mirror);
+//   O = 4: lane 7 - l within its 8 (DPP row_half_mirror);
+//   O = 2, 1: lane ^ O (DPP quad_perm).
+// v_permlane32_swap(x, y) returns {x with lanes 32-63 replaced by lanes
+// 0-31 of y, y with lanes 0-31 replaced by lanes 32-63 of x};
+// v_permlane16_swap does the same between the odd rows of x and the even
+// rows of y.
+template <int CTRL>
+__device__ inline double dpp_mov(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0,
+            (int)__double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0,
+            (int)__double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Value of the partner lane at level O <= 8.
+template <int O>
+__device__ inline double lane_partner(double v)
+{
+    static_assert(O == 8 || O == 4 || O == 2 || O == 1, "level");
+    constexpr int ctrl = O == 8 ? 0x140 : O == 4 ? 0x141 :
+        O == 2 ? 0x4e : 0xb1;
+    return dpp_mov<ctrl>(v);
+}
+
+// For O = 32 or 16: a lane of the lower half (O = 32) or of an even row
+// (O = 16) gets its x plus its partner's x, a lane of the other half its
+// partner's y plus its y; the value of the lower (even) lane is always the
+// left operand.
+template <int O>
+__device__ inline double swap_sum(double x, double y)
+{
+    static_assert(O == 32 || O == 16, "level");
+    const unsigned xl = (unsigned)__double2loint(x);
+    const unsigned xh = (unsigned)__double2hiint(x);
+    const unsigned yl = (unsigned)__double2loint(y);
+    const unsigned yh = (unsigned)__double2hiint(y);
+    unsigned a0, a1, b0, b1;
+    if constexpr(O == 32)
+    {
+        const auto l = __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
+        const auto h = __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
+        a0 = l[0]; b0 = l[1]; a1 = h[0]; b1 = h[1];
+    }
+    else
+    {
+        const auto l = __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
+        const auto h = __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
+        a0 = l[0]; b0 = l[1]; a1 = h[0]; b1 = h[1];
+    }
+    return __hiloint2double((int)a1, (int)a0)+__hiloint2double((int)b1,
+            (int)b0);
+}
+
+// v plus the partner's v at level O; both partners get the same sum.
+template <int O>
+__device__ inline double pair_sum(double v)
+{
+    if constexpr(O >= 16)
+        return swap_sum<O>(v, v);
+    else
+        return v+lane_partner<O>(v);
+}
+
+// Sum over an aligned group of g lanes (g a power of two, at most 64, the
+// same in the whole wave); every lane of the group gets the same total.
+__device__ inline double group_sum_lanes(double v, int g)
+{
+    if(g >= 64)
+        v = pair_sum<32>(v);
+    if(g >= 32)
+        v = pair_sum<16>(v);
+    if(g >= 16)
+        v = pair_sum<8>(v);
+    if(g >= 8)
+        v = pair_sum<4>(v);
+    if(g >= 4)
+        v = pair_sum<2>(v);
+    if(g >= 2)
+        v = pair_sum<1>(v);
+    return v;
+}
+
 // Byte transpose of four words: p[s] holds byte s of z0, z1, z2, z3 (in
 // that order from the low byte).  v_perm_b32 selector bytes 0-3 pick bytes
 // of the second operand, 4-7 bytes of the first.

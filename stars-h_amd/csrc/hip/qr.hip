@@ -20,10 +20,7 @@ namespace {
 
 __device__ inline double wave_sum(double v)
 {
-#pragma unroll
-    for(int o = 32; o > 0; o >>= 1)
-        v += __shfl_xor(v, o, 64);
-    return v;
+    return group_sum_lanes(v, 64);
 }
 
 // Column norms squared: which = 0 -> Q buffer (m x k), 2 -> Bt (n x kc).
@@ -48,23 +45,36 @@ __global__ __launch_bounds__(256) void colnorm_kernel(const TileDesc *tiles,
 }
 
 // Sum over the 64 lanes of a wave of each of V values (V a power of two,
-// at most 64) by recursive halving: at every step a lane keeps half of its
-// values and adds its partner's copy of them, so V-1+log2(64/V) shuffles
-// reduce all V.  On return the lane holds the total of value index
-// halving_index<V>(lane).  The steps are unrolled by template recursion.
+// at most 64) by recursive halving: at every level O (32, 16, ..., 1) a
+// lane keeps half of its values, the upper half if lane & O, and adds its
+// partner's copy of them (partners as in lane_partner / swap_sum), so
+// V-1+log2(64/V) exchanges reduce all V.  On return the lane holds the
+// total of value index halving_index<V>(lane).  The steps are unrolled by
+// template recursion.
 template <int H, int O>
 struct Halve
 {
     template <int V>
     static __device__ inline void run(double (&v)[V], int lane)
     {
-        const bool up = (lane & O) != 0;
-#pragma unroll
-        for(int i = 0; i < H; i++)
+        if constexpr(O >= 16)
         {
-            const double send = up ? v[i] : v[i+H];
-            const double keep = up ? v[i+H] : v[i];
-            v[i] = keep+__shfl_xor(send, O, 64);
+            // the swap hands the lower (even-row) lane's v[i] and the
+            // upper lane's v[i+H] to the partner that keeps them
+#pragma unroll
+            for(int i = 0; i < H; i++)
+                v[i] = swap_sum<O>(v[i], v[i+H]);
+        }
+        else
+        {
+            const bool up = (lane & O) != 0;
+#pragma unroll
+            for(int i = 0; i < H; i++)
+            {
+                const double send = up ? v[i] : v[i+H];
+                const double keep = up ? v[i+H] : v[i];
+                v[i] = keep+lane_partner<O>(send);
+            }
         }
         Halve<H/2, O/2>::run(v, lane);
     }
@@ -77,9 +87,11 @@ struct Halve<0, O>
     static __device__ inline void run(double (&v)[V], int lane)
     {
         (void)lane;
-#pragma unroll
-        for(int o = O; o >= 1; o >>= 1)
-            v[0] += __shfl_xor(v[0], o, 64);
+        if constexpr(O >= 1)
+        {
+            v[0] = pair_sum<O>(v[0]);
+            Halve<0, O/2>::run(v, lane);
+        }
     }
 };
 
