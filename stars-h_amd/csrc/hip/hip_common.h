@@ -213,74 +213,114 @@ __device__ inline double kval_pt(const KernelParams &kp,
     return kval_of_dist<FAM>(kp, dist);
 }
 
-// exp(x) for finite x <= 0 to a relative error below 2^-43, enough for the
-// int8 paths, which round every entry to 2^-31 of its chunk's largest one.
-// Table-driven method of P. T. P. Tang ("Table-driven implementation of the
-// exponential function in IEEE floating-point arithmetic", ACM TOMS 15(2),
-// 1989): x = (64 N + j) ln2/64 + r with |r| <= ln2/128, so exp(x) =
-// 2^N 2^(j/64) e^r, with e^r - 1 from its Taylor polynomial of degree 4
-// (remainder below 2^-44).  The nearest integer to 64 x/ln2 is read off the
-// low word of 1.5 2^52 + 64 x/ln2.  tab[j] = 2^(j/64), j = 0..63
-// (exp2_table).  Arguments below -1000 give 0, like exp.
-__device__ inline double exp_nonpos(double x, const double *tab)
+// s exp(x) for finite x <= 0 to a relative error below 2^-44, enough for
+// the int8 paths, which round every entry to 2^-31 of its chunk's largest
+// one.  Table-driven method of P. T. P. Tang ("Table-driven implementation
+// of the exponential function in IEEE floating-point arithmetic", ACM TOMS
+// 15(2), 1989): x = (1024 N + j) ln2/1024 + r with |r| <= ln2/2048, so
+// exp(x) = 2^N 2^(j/1024) e^r, with e^r - 1 from its Taylor polynomial of
+// degree 3 (remainder below r^4/24 < 2^-50).  The nearest integer to
+// 1024 x/ln2 is read off the low word of 1.5 2^52 + 1024 x/ln2.  The
+// reduction uses one constant, ln2/1024 rounded to a double (off by less
+// than 2.3e-20): for x >= -1000, |1024 N + j| < 1.48e6 and r is off by
+// less than 3.4e-14.  tab[j] = s 2^(j/1024), j = 0..1023 (exp_table), so
+// the factor s costs nothing per entry.  Arguments below -1000 give 0, like
+// exp.
+constexpr int EXP_TAB = 1024;
+
+__device__ inline double sexp_nonpos(double x, const double *tab)
 {
     constexpr double shift = 0x1.8p52;
     x = x > -1000.0 ? x : -1000.0;
-    const double sh = fma(x, 92.33248261689366, shift);    // 64/ln2
+    const double sh = fma(x, 1477.3197218702985, shift);    // 1024/ln2
     const int n = (int)__double_as_longlong(sh);
     const double nd = sh-shift;
-    double r = fma(-nd, 0.010830424696249145, x);        // ln2/64, high part
-    r = fma(-nd, 3.623510646634843e-19, r);              // low part
-    double p = fma(r, 1.0/24.0, 1.0/6.0);
-    p = fma(p, r, 0.5);
+    const double r = fma(-nd, 0.0006769015435155716, x);    // ln2/1024
+    double p = fma(r, 1.0/6.0, 0.5);
     p = fma(p, r, 1.0);
     p *= r;
-    const double t = tab[n & 63];
-    return ldexp(fma(t, p, t), n >> 6);
+    const double t = tab[n & (EXP_TAB-1)];
+    return ldexp(fma(t, p, t), n >> 10);
 }
 
-// Fills tab[0..63] with 2^(j/64); called by the first 64 threads.
-__device__ inline void exp2_table(double *tab, int tid)
+// Fills tab[0..EXP_TAB-1] with s 2^(j/1024); called by all nthreads
+// threads of the block.
+__device__ inline void exp_table(double *tab, double s, int tid,
+        int nthreads)
 {
-    if(tid < 64)
-        tab[tid] = exp2((double)tid*(1.0/64.0));
+    for(int j = tid; j < EXP_TAB; j += nthreads)
+        tab[j] = s*exp2((double)j*(1.0/EXP_TAB));
 }
 
-// kval_of_dist with exp_nonpos for the exponential families (exp, sqexp,
-// Matern); the other families are unchanged.
 template <int FAM>
-__device__ inline double kval_of_dist_tab(const KernelParams &kp, double dist,
+constexpr bool exp_family()
+{
+    return FAM == STARSH_HIP_EXP || FAM == STARSH_HIP_SQREXP ||
+        FAM == STARSH_HIP_MATERN || FAM == STARSH_HIP_MATERN2;
+}
+
+// Factor c applied to the point coordinates in the int8 paths: with
+// d = |c (x_i - y_j)|^2 the exponent of the exponential families is -d
+// (sqexp, c = 1/(sqrt(2) beta)), -sqrt(d) (exp, c = 1/beta) and -sqrt(d)
+// times the Matern polynomial (c = theta or 1/beta); 1 for the others.
+template <int FAM>
+__device__ inline double coord_scale(const KernelParams &kp)
+{
+    switch(FAM)
+    {
+        case STARSH_HIP_EXP:
+            return -kp.ibexp;
+        case STARSH_HIP_SQREXP:
+            return sqrt(-kp.ibsqr);
+        case STARSH_HIP_MATERN:
+            return kp.theta;
+        case STARSH_HIP_MATERN2:
+            return kp.ibeta;
+        default:
+            return 1.0;
+    }
+}
+
+// Entry A(i, j) of the int8 paths from d = |c (x_i - y_j)|^2
+// (c = coord_scale): the exponential families through sexp_nonpos with
+// sigma in the table (exp_table(tab, sigma, ...)), computed for every
+// distance and selected afterwards, so that the compiler keeps the
+// evaluation free of branches.  At d = 0 they give sigma by themselves
+// (exp(0) = 1, Matern polynomial 1): the select of sigma + noise is made
+// only with NOISE (some problem of the batch has a nonzero noise).  The
+// other families are kval_of_dist (c = 1).
+template <int FAM, bool NOISE>
+__device__ inline double kval_scaled(const KernelParams &kp, double d,
         const double *tab)
 {
-    // computed for every distance and selected afterwards, so that the
-    // compiler keeps the evaluation free of branches
     double v;
     switch(FAM)
     {
         case STARSH_HIP_EXP:
-            v = kp.sigma*exp_nonpos(sqrt(dist)*kp.ibexp, tab);
+            v = sexp_nonpos(-sqrt(d), tab);
             break;
         case STARSH_HIP_SQREXP:
-            v = kp.sigma*exp_nonpos(dist*kp.ibsqr, tab);
+            v = sexp_nonpos(-d, tab);
             break;
         case STARSH_HIP_MATERN:
         case STARSH_HIP_MATERN2:
         {
-            const double x = sqrt(dist)*(FAM == STARSH_HIP_MATERN ?
-                    kp.theta : kp.ibeta);
-            const double e = exp_nonpos(-x, tab);
+            const double x = sqrt(d);
+            const double e = sexp_nonpos(-x, tab);
             // 1, 1 + x or 1 + x + x^2/3 for nu = 0.5, 1.5, 2.5, as one
             // polynomial whose coefficients do not depend on the entry
             const double c1 = kp.nu == 0.5 ? 0.0 : 1.0;
             const double c2 = kp.nu == 0.5 || kp.nu == 1.5 ? 0.0 : 1.0/3.0;
-            const double f = fma(x, fma(x, c2, c1), 1.0);
-            v = kp.sigma*(f*e);
+            v = fma(x, fma(x, c2, c1), 1.0)*e;
             break;
         }
         default:
-            return kval_of_dist<FAM>(kp, dist);
+            return kval_of_dist<FAM>(kp, d);
     }
-    return dist == 0 ? kp.sigma+kp.noise : v;
+    if constexpr(NOISE)
+        return d == 0 ? kp.sigma+kp.noise : v;
+    else
+        return v;
 }
 
 template <int NDIM, int FAM>

@@ -18,6 +18,7 @@ struct GemmArgs
     double *out;          // output base for U / V
     const double *abase;  // explicit tiles base (dense mode)
     int ndim, family;     // kernel of every tile of the batch
+    int noisy;            // some problem of the batch has a nonzero noise
 };
 
 int pipeline_gemm(int mode, const TileDesc *d_tiles, int ntiles, int mmax,

@@ -387,6 +387,8 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     g.abase = in->d_a;
     g.ndim = kps[q0].ndim;
     g.family = kps[q0].family;
+    for(const auto &kp : kps)
+        g.noisy |= kp.noise != 0.0;
 
     hipEvent_t ev[4];
     for(int e = 0; e < 4; e++)

@@ -93,21 +93,23 @@ constexpr int proj_waves()
 // are staged in the wave's LDS area one chunk ahead.  The digit planes of
 // Q are read from global memory (all blocks of a tile run on one XCD and
 // share them through its L2).
-template <int NDIM, int FAM, int CB>
+template <int NDIM, int FAM, int CB, bool NOISE>
 __global__ __launch_bounds__(256, (proj_waves<NDIM, FAM, CB>()))
 void proj_i8_kernel(const TileDesc *tiles, double *ws,
         const signed char *qd, int ntiles, int gnbm, int gnbn)
 {
     constexpr int NF = CB/16;
     __shared__ __align__(16) double xs[4][2][NDIM][KB];
-    __shared__ double tab[64];
+    __shared__ double tab[exp_family<FAM>() ? EXP_TAB : 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    exp2_table(tab, tid);
-    __syncthreads();
     int tile, bm, bn;
     if(!map_block(blockIdx.x, ntiles, gnbm, gnbn, false, tile, bm, bn))
-        return;
+        return;     // the whole block
     const TileDesc t = tiles[tile];
+    if constexpr(exp_family<FAM>())
+        exp_table(tab, t.kp.sigma, tid, 256);
+    __syncthreads();
+    const double cs = coord_scale<FAM>(t.kp);
     const int N = t.kc;
     const int j0 = bm*64+wave*16, c0 = bn*CB;
     if(t.m <= 0 || t.n <= 0 || N <= 0 || c0 >= N || j0 >= t.n)
@@ -119,7 +121,7 @@ void proj_i8_kernel(const TileDesc *tiles, double *ws,
     double yj[NDIM];
 #pragma unroll
     for(int d = 0; d < NDIM; d++)
-        yj[d] = t.xc[d*t.nc+t.c0+jj];
+        yj[d] = cs*t.xc[d*t.nc+t.c0+jj];
     // columns up to k16 - 1 exist in the digit planes (the live fragments
     // end there); those from kc on hold no digits and give unstored sums
     const int m64 = (t.m+63)/64*64, k16 = (t.k+15)/16*16;
@@ -128,7 +130,8 @@ void proj_i8_kernel(const TileDesc *tiles, double *ws,
     // row points of chunk 0: lane l holds row l of the chunk
 #pragma unroll
     for(int d = 0; d < NDIM; d++)
-        xs[wave][0][d][lane] = t.xr[d*t.nr+t.r0+(lane < t.m ? lane : t.m-1)];
+        xs[wave][0][d][lane] = cs*t.xr[d*t.nr+t.r0+(lane < t.m ? lane :
+                t.m-1)];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -154,7 +157,7 @@ void proj_i8_kernel(const TileDesc *tiles, double *ws,
         const int in = i0+KB+lane;
 #pragma unroll
         for(int d = 0; d < NDIM; d++)
-            xn[d] = t.xr[d*t.nr+t.r0+(in < t.m ? in : t.m-1)];
+            xn[d] = cs*t.xr[d*t.nr+t.r0+(in < t.m ? in : t.m-1)];
         // ---- A^T entries (j, i0 + 16 ig + e); rows past the tile repeat
         // the last row's point (no effect on the maximum) and meet zero
         // digits of Q; a j past the tile only feeds unstored outputs
@@ -170,7 +173,7 @@ void proj_i8_kernel(const TileDesc *tiles, double *ws,
                 const double q = xs[wave][buf][d][16*ig+e]-yj[d];
                 dist = fma(q, q, dist);
             }
-            a[e] = kval_of_dist_tab<FAM>(t.kp, dist, tab);
+            a[e] = kval_scaled<FAM, NOISE>(t.kp, dist, tab);
             mx = fmax(mx, fabs(a[e]));
         }
         mx = fmax(mx, __shfl_xor(mx, 16, 64));
@@ -288,7 +291,7 @@ struct ProjLaunch
     const GemmArgs &g;
     const signed char *qd;
     hipStream_t s;
-    template <int NDIM, int FAM> int operator()() const
+    template <int NDIM, int FAM, bool NOISE> int launch() const
     {
         const int nbm = (nmax+63)/64;
         if(kmax <= 64)
@@ -296,7 +299,7 @@ struct ProjLaunch
             const int64_t grid = mapped_grid(ntiles, nbm, 1, false);
             if(grid > 0x7fffffff)
                 return STARSH_WRONG_PARAMETER;
-            hipLaunchKernelGGL((proj_i8_kernel<NDIM, FAM, 64>),
+            hipLaunchKernelGGL((proj_i8_kernel<NDIM, FAM, 64, NOISE>),
                     dim3((unsigned)grid), dim3(256), 0, s, tiles, g.ws, qd,
                     ntiles, nbm, 1);
         }
@@ -306,12 +309,20 @@ struct ProjLaunch
             const int64_t grid = mapped_grid(ntiles, nbm, nbn, false);
             if(grid > 0x7fffffff)
                 return STARSH_WRONG_PARAMETER;
-            hipLaunchKernelGGL((proj_i8_kernel<NDIM, FAM, 128>),
+            hipLaunchKernelGGL((proj_i8_kernel<NDIM, FAM, 128, NOISE>),
                     dim3((unsigned)grid), dim3(256), 0, s, tiles, g.ws, qd,
                     ntiles, nbm, nbn);
         }
         HIP_TRY(hipGetLastError());
         return STARSH_SUCCESS;
+    }
+    template <int NDIM, int FAM> int operator()() const
+    {
+        // the select of sigma + noise at distance 0 only where a problem
+        // of the batch has a nonzero noise (other families always select)
+        if(exp_family<FAM>() && !g.noisy)
+            return launch<NDIM, FAM, false>();
+        return launch<NDIM, FAM, true>();
     }
 };
 

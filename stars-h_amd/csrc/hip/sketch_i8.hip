@@ -77,7 +77,7 @@ constexpr int sketch_waves()
     return CB == 64 && NDIM < 4 ? 3 : 2;
 }
 
-template <int NDIM, int FAM, int CB>
+template <int NDIM, int FAM, int CB, bool NOISE>
 __global__ __launch_bounds__(256, (sketch_waves<NDIM, FAM, CB>()))
 void sketch_i8_kernel(const TileDesc *tiles,
         double *ws, const signed char *om8)
@@ -86,11 +86,13 @@ void sketch_i8_kernel(const TileDesc *tiles,
     __shared__ __align__(16) signed char As[4][4][16*APAD];
     __shared__ __align__(16) double xs[4][2][NDIM][KB];
     __shared__ double csc[4][16];
-    __shared__ double tab[64];
+    __shared__ double tab[exp_family<FAM>() ? EXP_TAB : 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    exp2_table(tab, tid);
-    __syncthreads();
     const TileDesc t = tiles[blockIdx.y];
+    if constexpr(exp_family<FAM>())
+        exp_table(tab, t.kp.sigma, tid, 256);
+    __syncthreads();
+    const double cs = coord_scale<FAM>(t.kp);
     const int nbm = (t.m+RB-1)/RB;
     const int bm = blockIdx.x % nbm, bn = blockIdx.x / nbm;
     const int row0 = bm*RB+wave*16, col0 = bn*CB;
@@ -101,7 +103,7 @@ void sketch_i8_kernel(const TileDesc *tiles,
     double xi[NDIM];
 #pragma unroll
     for(int d = 0; d < NDIM; d++)
-        xi[d] = row_ok ? t.xr[d*t.nr+t.r0+row0+gi] : 0.0;
+        xi[d] = row_ok ? cs*t.xr[d*t.nr+t.r0+row0+gi] : 0.0;
     const signed char *om = om8+t.om8_off;
     const int ldo = (t.n+63)/64*64;
     signed char *Aw[4] = {As[wave][0], As[wave][1], As[wave][2],
@@ -110,7 +112,7 @@ void sketch_i8_kernel(const TileDesc *tiles,
     const int64_t cbase = t.c0;
 #pragma unroll
     for(int d = 0; d < NDIM; d++)
-        xs[wave][0][d][lane] = t.xc[d*t.nc+cbase+(lane < t.n ? lane :
+        xs[wave][0][d][lane] = cs*t.xc[d*t.nc+cbase+(lane < t.n ? lane :
                 t.n-1)];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -140,7 +142,7 @@ void sketch_i8_kernel(const TileDesc *tiles,
         const int kn = k0+KB+lane;
 #pragma unroll
         for(int d = 0; d < NDIM; d++)
-            xn[d] = t.xc[d*t.nc+cbase+(kn < t.n ? kn : t.n-1)];
+            xn[d] = cs*t.xc[d*t.nc+cbase+(kn < t.n ? kn : t.n-1)];
         // ---- A: FP64 entries of the chunk, scale, four byte planes
         double a[16];
         double mx = 0.0;
@@ -158,7 +160,7 @@ void sketch_i8_kernel(const TileDesc *tiles,
             // row, which is not stored, and a column past the tile repeats
             // the last column's point (so it leaves the chunk maximum
             // unchanged) and meets a zero row of Omega
-            a[e] = kval_of_dist_tab<FAM>(t.kp, dist, tab);
+            a[e] = kval_scaled<FAM, NOISE>(t.kp, dist, tab);
             mx = fmax(mx, fabs(a[e]));
         }
         mx = fmax(mx, __shfl_xor(mx, 1, 64));
@@ -273,15 +275,24 @@ struct SketchLaunch
     const GemmArgs &g;
     const signed char *om8;
     hipStream_t s;
-    template <int NDIM, int FAM> int operator()() const
+    template <int NDIM, int FAM, bool NOISE> void launch() const
     {
         if(kmax <= 64)
-            hipLaunchKernelGGL((sketch_i8_kernel<NDIM, FAM, 64>),
+            hipLaunchKernelGGL((sketch_i8_kernel<NDIM, FAM, 64, NOISE>),
                     dim3(nbm, ntiles), dim3(256), 0, s, tiles, g.ws, om8);
         else
-            hipLaunchKernelGGL((sketch_i8_kernel<NDIM, FAM, 128>),
+            hipLaunchKernelGGL((sketch_i8_kernel<NDIM, FAM, 128, NOISE>),
                     dim3(nbm*((kmax+127)/128), ntiles), dim3(256), 0, s,
                     tiles, g.ws, om8);
+    }
+    template <int NDIM, int FAM> int operator()() const
+    {
+        // the select of sigma + noise at distance 0 only where a problem
+        // of the batch has a nonzero noise (other families always select)
+        if(exp_family<FAM>() && !g.noisy)
+            launch<NDIM, FAM, false>();
+        else
+            launch<NDIM, FAM, true>();
         HIP_TRY(hipGetLastError());
         return STARSH_SUCCESS;
     }

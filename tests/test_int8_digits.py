@@ -79,36 +79,46 @@ def _fma(a, b, c):
     return (L(a) * L(b) + L(c)).astype(np.float64)
 
 
-def _exp_nonpos(x):
-    """Numpy restatement of exp_nonpos (stars-h_amd/csrc/hip/hip_common.h):
-    Tang's table-driven exp with 64 table entries and a degree-4 Taylor
-    polynomial; fma emulated in extended precision."""
+def _sexp_nonpos(x, s=1.0):
+    """Numpy restatement of sexp_nonpos (stars-h_amd/csrc/hip/hip_common.h):
+    Tang's table-driven exp with 1024 table entries s*2^(j/1024), a
+    degree-3 Taylor polynomial and a one-constant reduction; fma emulated
+    in extended precision."""
     x = np.maximum(x, -1000.0)
     shift = 1.5 * 2.0**52
-    sh = _fma(x, 92.33248261689366, shift)
+    sh = _fma(x, 1477.3197218702985, shift)
     n = sh.view(np.int64).astype(np.int32)         # low 32 bits
     nd = sh - shift
-    r = _fma(-nd, 0.010830424696249145, x)
-    r = _fma(-nd, 3.623510646634843e-19, r)
-    p = _fma(r, 1.0 / 24.0, 1.0 / 6.0)
-    for c in (0.5, 1.0):
-        p = _fma(p, r, c)
+    r = _fma(-nd, 0.0006769015435155716, x)
+    p = _fma(r, 1.0 / 6.0, 0.5)
+    p = _fma(p, r, 1.0)
     p = p * r
-    tab = np.exp2(np.arange(64) / 64.0)
-    t = tab[n & 63]
-    return np.ldexp(_fma(t, p, t), (n >> 6).astype(np.int32))
+    tab = s * np.exp2(np.arange(1024) / 1024.0)
+    t = tab[n & 1023]
+    return np.ldexp(_fma(t, p, t), (n >> 10).astype(np.int32))
 
 
 def test_table_exp_accuracy():
     rng = np.random.default_rng(3)
-    x = np.concatenate([-rng.uniform(0, 745, 200000),
+    x = np.concatenate([-rng.uniform(0, 1000, 200000),
                         -rng.uniform(0, 1, 20000), np.array([0.0, -1e-300])])
-    got = _exp_nonpos(x)
+    got = _sexp_nonpos(x)
     want = np.exp(x.astype(np.longdouble))
     ok = want > 1e-300          # normal range
     rel = np.abs((got[ok] - want[ok]) / want[ok]).astype(np.float64)
-    assert rel.max() <= 2.0**-43
-    assert _exp_nonpos(np.array([-1e6, -1000.0]))[0] == 0.0
+    assert rel.max() <= 2.0**-44
+    assert _sexp_nonpos(np.array([-1e6, -1000.0]))[0] == 0.0
+
+
+def test_table_exp_folded_factor():
+    """The factor in the table gives s*exp(x), exactly s at x = 0 (the
+    value at distance 0 without a select)."""
+    s = 1.7
+    assert _sexp_nonpos(np.array([0.0, -0.0]), s).tolist() == [s, s]
+    x = -np.linspace(0, 50, 1001)
+    want = s * np.exp(x.astype(np.longdouble))
+    rel = np.abs((_sexp_nonpos(x, s) - want) / want).astype(np.float64)
+    assert rel.max() <= 2.0**-44
 
 
 def test_scaled_byte_digits_rounding():
