@@ -59,6 +59,55 @@ __device__ inline bool rotate_pair(double *wa, double *wb, int p, int g,
     return true;
 }
 
+// rotate_pair with both columns read into registers once (E rows per
+// thread, p <= E g): the loads of a rotation are issued together and the
+// update reads no memory.  The same sums in the same order (rows past p
+// hold zeros, which leave every sum unchanged), so the same rotation.
+template <int E>
+__device__ inline bool rotate_pair_reg(double *wa, double *wb, int p, int g,
+        int gl, double ctol)
+{
+    double u[E], v[E];
+    double al = 0.0, be = 0.0, ga = 0.0;
+#pragma unroll
+    for(int t = 0; t < E; t++)
+    {
+        const int i = gl+t*g;
+        u[t] = i < p ? wa[i] : 0.0;
+        v[t] = i < p ? wb[i] : 0.0;
+    }
+#pragma unroll
+    for(int t = 0; t < E; t++)
+    {
+        al += u[t]*u[t];
+        be += v[t]*v[t];
+        ga += u[t]*v[t];
+    }
+    al = group_sum(al, g);
+    be = group_sum(be, g);
+    ga = group_sum(ga, g);
+    if(!(al > 0.0 && be > 0.0 && fabs(ga) > ctol*sqrt(al*be)))
+        return false;
+    double zeta = (be-al)/(2.0*ga);
+    double t;
+    if(fabs(zeta) > 1e150)
+        t = 0.5/zeta;
+    else
+        t = (zeta >= 0.0 ? 1.0 : -1.0)/(fabs(zeta)+sqrt(1.0+zeta*zeta));
+    double c = 1.0/sqrt(1.0+t*t), s = c*t;
+#pragma unroll
+    for(int q = 0; q < E; q++)
+    {
+        const int i = gl+q*g;
+        if(i < p)
+        {
+            wa[i] = c*u[q]-s*v[q];
+            wb[i] = s*u[q]+c*v[q];
+        }
+    }
+    return true;
+}
+
 // Columns of up to CREG*64 rows held in registers by one wave (lane l holds
 // rows l, l+64, ...).
 constexpr int CREG = 16;
@@ -151,8 +200,12 @@ __device__ void jacobi(double *W, int ldw, int p, bool in_lds,
                 }
                 if(b >= p)
                     continue;
-                if(rotate_pair(W+(int64_t)a*ldw, W+(int64_t)b*ldw, p, g, gl,
-                            ctol) && gl == 0)
+                double *wa = W+(int64_t)a*ldw, *wb = W+(int64_t)b*ldw;
+                const bool rot = p <= 4*g ?
+                    rotate_pair_reg<4>(wa, wb, p, g, gl, ctol) :
+                    p <= 8*g ? rotate_pair_reg<8>(wa, wb, p, g, gl, ctol) :
+                    rotate_pair(wa, wb, p, g, gl, ctol);
+                if(rot && gl == 0)
                     atomicAdd(&rotations, 1);
             }
             __syncthreads();
@@ -373,8 +426,8 @@ __device__ void jacobi_blocked32(double *W, int p, double *L, int *sweeps_out)
                         a = b;
                         b = tmp;
                     }
-                    if(b < wi && rotate_pair(L+(int64_t)a*p, L+(int64_t)b*p,
-                                p, 32, gl, ctol) && gl == 0)
+                    if(b < wi && rotate_pair_reg<CR32>(L+(int64_t)a*p,
+                                L+(int64_t)b*p, p, 32, gl, ctol) && gl == 0)
                         atomicAdd(&rotations, 1);
                 }
                 __syncthreads();
