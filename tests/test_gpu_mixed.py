@@ -148,3 +148,29 @@ def test_mixed_exp2d_tight_tolerance_uses_fp64_sketch():
     assert pm.near_tiles() == pp.near_tiles()
     pp.free_matrix()
     pm.free_matrix()
+
+
+@pytest.mark.parametrize("kw", [
+    dict(kernel="exp", beta=0.1, sigma=2.0, noise=0.05),
+    dict(kernel="matern", beta=0.05, nu=0.5, sigma=1.5),
+    dict(kernel="matern2", beta=0.1, nu=2.5),
+])
+def test_mixed_sigma_noise_and_matern_variants(kw):
+    """The int8 paths fold sigma into the exp table, scale the points per
+    family and select sigma + noise at distance 0 only for batches with a
+    noise: the mixed form keeps the accuracy and the false far-field tiles
+    (the diagonal tiles with a nugget) of the plain form."""
+    _say(f"case: mixed vs plain, 2D {kw} N=6400 nb=800 tol 1e-7")
+    res = _mixed_vs_plain("spatial", 6400, 2, 800, 200, 16, 1e-7, seed=2,
+                          **kw)
+    rp, gp, _, _, tp, pp = res["HIP"]
+    rm, gm, _, _, tm, pm = res["HIP_MIXED"]
+    _say(f"  plain err {gp:.3e} mixed err {gm:.3e}; false far plain "
+         f"{len(pp.near_tiles())} mixed {len(pm.near_tiles())}")
+    assert tp == tm
+    assert pm.near_tiles() == pp.near_tiles()
+    assert gm <= 1e-7
+    assert abs(gm - gp) <= 0.05 * gp
+    assert np.max(np.abs(rm - rp)) <= 3
+    pp.free_matrix()
+    pm.free_matrix()
