@@ -80,7 +80,8 @@ template <int NDIM, int FAM, int CB>
 constexpr int proj_waves()
 {
     // three waves per SIMD fit without spills except for sin/cos
-    return CB == 64 && NDIM < 4 && FAM != STARSH_HIP_SIN &&
+    return CB == 64 && NDIM < 3 && exp_family<FAM>() ? 4 :
+        CB == 64 && NDIM < 4 && FAM != STARSH_HIP_SIN &&
         FAM != STARSH_HIP_COS ? 3 : 2;
 }
 
