@@ -112,10 +112,16 @@ struct ActiveProbe
 
 ActiveProbe probe;
 
-// Panel width of the block Gram-Schmidt (4, or 8 for tiles of at most
-// 2048 rows); set by starsh_hip_set_option("qr_panel").  Narrow panels
-// measured faster on MI355X: smaller blocks, more of them per CU.
-int qr_panel_width = 4;
+// Panel width of the block Gram-Schmidt: 0 (by batch, default), 4, or 8
+// (tiles of at most 2048 rows); set by starsh_hip_set_option("qr_panel").
+// A 4-column panel keeps two tiles resident per CU (512 on the chip), an
+// 8-column one a single tile, with half the launches and half the re-reads
+// of earlier columns.  Batches of up to PANEL8_TILES tiles are done in at
+// most three rounds of 4-column workgroups and take them (C2, 820 tiles:
+// 10.1 ms against 11.7 ms with 8 columns); larger ones take 8 columns (C3,
+// 2080 tiles: 1.61 s against 1.72 s in the plain form).
+int qr_panel_width = 0;
+constexpr int PANEL8_TILES = 1536;
 
 // Projection of the mixed form on int8 digit planes (1, default) or FP64
 // MFMA (0); set by starsh_hip_set_option("proj_int8").
@@ -125,7 +131,9 @@ int bcgs2(TileDesc *d_tiles, int ntiles, int which, double *ws, int kmax,
         int width_max, int rows_max, int *d_keep, double tau_first,
         int *d_active, hipStream_t s)
 {
-    const int pb = rows_max <= 2048 && qr_panel_width != 4 ? 8 : 4;
+    const bool wide = qr_panel_width == 8 ||
+        (qr_panel_width == 0 && ntiles > PANEL8_TILES);
+    const int pb = rows_max <= 2048 && wide ? 8 : 4;
     TRY(probe.init());
     TRY(pipeline_count_active(d_tiles, ntiles, which, d_active, s));
     int panel = 0;
@@ -580,7 +588,7 @@ extern "C" int starsh_hip_proj_int8(int on)
 
 extern "C" int starsh_hip_qr_panel(int width)
 {
-    if(width != 4 && width != 8)
+    if(width != 0 && width != 4 && width != 8)
         return STARSH_WRONG_PARAMETER;
     qr_panel_width = width;
     return STARSH_SUCCESS;
