@@ -25,6 +25,31 @@ __device__ inline double group_sum(double v, int g)
     return group_sum_lanes(v, g);
 }
 
+// Rotation (c, s) that makes two columns with squared norms al, be and
+// inner product ga orthogonal: Rutishauser's t = tan(angle) =
+// sign(zeta)/(|zeta| + sqrt(1 + zeta^2)), zeta = (be - al)/(2 ga), the
+// smaller root of t^2 + 2 zeta t - 1 = 0, written with d = be - al and
+// h = 2 ga as t = sign(d) h/(|d| + sqrt(d^2 + h^2)), so that one division
+// and one square root do; c = 1/sqrt(1 + t^2) from the reciprocal square
+// root.  Operands beyond 1e150 (whose squares would overflow) take the
+// zeta form.
+__device__ inline void jacobi_rotation(double al, double be, double ga,
+        double &c, double &s)
+{
+    const double d = be-al, h = 2.0*ga;
+    double t;
+    if(fabs(d) < 1e150 && fabs(h) < 1e150)
+        t = (d >= 0.0 ? h : -h)/(fabs(d)+sqrt(fma(d, d, h*h)));
+    else
+    {
+        const double zeta = d/h;
+        t = fabs(zeta) > 1e150 ? 0.5/zeta : (zeta >= 0.0 ? 1.0 : -1.0)/
+            (fabs(zeta)+sqrt(1.0+zeta*zeta));
+    }
+    c = rsqrt(fma(t, t, 1.0));
+    s = c*t;
+}
+
 // One-sided Jacobi step on the column pair (wa, wb) of length p by a group
 // of g threads (gl = index in the group): returns true if it rotated.
 __device__ inline bool rotate_pair(double *wa, double *wb, int p, int g,
@@ -43,13 +68,8 @@ __device__ inline bool rotate_pair(double *wa, double *wb, int p, int g,
     ga = group_sum(ga, g);
     if(!(al > 0.0 && be > 0.0 && fabs(ga) > ctol*sqrt(al*be)))
         return false;
-    double zeta = (be-al)/(2.0*ga);
-    double t;
-    if(fabs(zeta) > 1e150)
-        t = 0.5/zeta;
-    else
-        t = (zeta >= 0.0 ? 1.0 : -1.0)/(fabs(zeta)+sqrt(1.0+zeta*zeta));
-    double c = 1.0/sqrt(1.0+t*t), s = c*t;
+    double c, s;
+    jacobi_rotation(al, be, ga, c, s);
     for(int i = gl; i < p; i += g)
     {
         double u = wa[i], v = wb[i];
@@ -88,13 +108,8 @@ __device__ inline bool rotate_pair_reg(double *wa, double *wb, int p, int g,
     ga = group_sum(ga, g);
     if(!(al > 0.0 && be > 0.0 && fabs(ga) > ctol*sqrt(al*be)))
         return false;
-    double zeta = (be-al)/(2.0*ga);
-    double t;
-    if(fabs(zeta) > 1e150)
-        t = 0.5/zeta;
-    else
-        t = (zeta >= 0.0 ? 1.0 : -1.0)/(fabs(zeta)+sqrt(1.0+zeta*zeta));
-    double c = 1.0/sqrt(1.0+t*t), s = c*t;
+    double c, s;
+    jacobi_rotation(al, be, ga, c, s);
 #pragma unroll
     for(int q = 0; q < E; q++)
     {
@@ -138,13 +153,8 @@ __device__ inline bool rotate_reg(double (&ca)[CREG], double *wb, int p,
     ga = group_sum(ga, 64);
     if(!(al > 0.0 && be > 0.0 && fabs(ga) > ctol*sqrt(al*be)))
         return false;
-    double zeta = (be-al)/(2.0*ga);
-    double t;
-    if(fabs(zeta) > 1e150)
-        t = 0.5/zeta;
-    else
-        t = (zeta >= 0.0 ? 1.0 : -1.0)/(fabs(zeta)+sqrt(1.0+zeta*zeta));
-    double c = 1.0/sqrt(1.0+t*t), s = c*t;
+    double c, s;
+    jacobi_rotation(al, be, ga, c, s);
 #pragma unroll
     for(int q = 0; q < CREG; q++)
     {
@@ -352,13 +362,8 @@ __device__ inline bool rotate_reg32(double (&ca)[CR32], double *wb, int p,
     ga = group_sum(ga, 32);
     if(!(al > 0.0 && be > 0.0 && fabs(ga) > ctol*sqrt(al*be)))
         return false;
-    double zeta = (be-al)/(2.0*ga);
-    double t;
-    if(fabs(zeta) > 1e150)
-        t = 0.5/zeta;
-    else
-        t = (zeta >= 0.0 ? 1.0 : -1.0)/(fabs(zeta)+sqrt(1.0+zeta*zeta));
-    double c = 1.0/sqrt(1.0+t*t), s = c*t;
+    double c, s;
+    jacobi_rotation(al, be, ga, c, s);
 #pragma unroll
     for(int q = 0; q < CR32; q++)
     {
