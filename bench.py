@@ -40,7 +40,7 @@ PEAK_FP64 = 78.6  # TFLOP/s, MI355X dense FP64 (vector and matrix)
 # FETCH_SIZE and WRITE_SIZE counters (rocprofv3 --pmc, one run per counter)
 # scaled by the same counters on a 1 GiB device copy (fetch x2.00, write
 # x1.00); commands and raw values in profiles/c2_sketch_hbm_bytes.md.
-C2_SKETCH_TRAFFIC = 7.355e9
+C2_SKETCH_TRAFFIC = 7.353e9
 
 WORKLOADS = {
     "C2": dict(kind="spatial", kernel="sqrexp", N=64000, ndim=2, nb=1600,
