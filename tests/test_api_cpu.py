@@ -227,3 +227,13 @@ def test_hip_driver_rejects_foreign_kernel_before_touching_gpu():
     foreign = KT(lambda *a: None)
     p.P.contents.kernel = C.cast(foreign, C.c_void_p)
     assert p.approximate(20, 1e-6, 0, fn) == S.STARSH_WRONG_PARAMETER
+
+
+def test_gram_schmidt_panel_option():
+    """qr_panel takes 0 (by batch size, the default), 4 or 8 and rejects
+    other widths; setting it touches no GPU."""
+    L = S.lib()
+    for w in (4, 8, 0):
+        assert L.starsh_hip_set_option(b"qr_panel", C.c_double(w)) == 0
+    assert L.starsh_hip_set_option(b"qr_panel", C.c_double(5)) != 0
+    assert L.starsh_hip_set_option(b"qr_panel", C.c_double(0)) == 0
