@@ -129,6 +129,8 @@ def run_form(S, w, backend, steps, warmup, autotune):
         schedule = {"first_bucket_width": [32, 64, 128, 256, 512][first],
                     "candidate_device_seconds": [round(x, 6) for x in secs]}
     dev, wall, dom = [], [], []
+    parts = ("prepare", "batch", "d2h", "scatter", "finish")
+    tl = {k: [] for k in parts}
     for it in range(warmup + steps):
         if it > 0:
             # a compression rewrites the format when it finds false
@@ -141,6 +143,8 @@ def run_form(S, w, backend, steps, warmup, autotune):
             dev.append(st.device_seconds)
             wall.append(st.wall_seconds)
             dom.append(st.dominant_kernel_seconds)
+            for k in parts:
+                tl[k].append(getattr(st, k + "_seconds"))
         if it < warmup + steps - 1:
             for p in ps:
                 p.free_matrix()
@@ -156,6 +160,13 @@ def run_form(S, w, backend, steps, warmup, autotune):
         "device_seconds": t,
         "step_device_seconds": [round(x, 6) for x in dev],
         "api_wall_seconds": sum(wall) / len(wall),
+        "step_api_wall_seconds": [round(x, 6) for x in wall],
+        # host-side timeline of the API call, mean over the timed steps:
+        # argument checks + point upload + U/V malloc, device batches (wall),
+        # pack + D2H of U/V, copy into the caller's buffers, format rewrite
+        # + near tiles + blrm
+        "api_timeline_seconds": {k: round(sum(v) / len(v), 6)
+                                 for k, v in tl.items()},
         # summed over the launches of one step
         "dominant_kernel_seconds": sum(dom) / len(dom),
         "dominant_kernel_launches": int(st.dominant_kernel_launches),

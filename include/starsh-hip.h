@@ -87,6 +87,13 @@ typedef struct starsh_hip_stats
     double rank_avg;
     double flops_algorithmic;   /* work of the reference algorithm */
     double sketch_bytes_or_flops; /* algorithmic flops of the sketch kernel */
+    /* host-side timeline of the call (seconds; their sum with the device
+     * batches is the wall time) */
+    double prepare_seconds;  /* argument checks, point upload, U/V malloc */
+    double batch_seconds;    /* wall time inside the device batches */
+    double d2h_seconds;      /* pack + device-to-host copy of U and V */
+    double scatter_seconds;  /* host copy into the caller's buffers */
+    double finish_seconds;   /* false far-field rewrite, near tiles, blrm */
 } starsh_hip_stats;
 
 int starsh_hip_get_stats(starsh_hip_stats *stats);
@@ -146,6 +153,14 @@ int starsh_hip_fp64_probe(double *mfma_tflops, double *valu_tflops);
  *   "proj_int8"           projection B^T = A^T Q of the mixed form on
  *                         int8 digit planes of both operands (1, default)
  *                         or on FP64 MFMA (0);
+ *   "jacobi_max_sweeps"   sweep limit of the Jacobi core SVD, 1..60 (0:
+ *                         the default 60); a tile whose core has not
+ *                         converged within it is a false far-field tile
+ *                         (rank -1), as for a failed dgesdd in
+ *                         src/backends/sequential/dense/drsdd.c:78;
+ *   "inject_failure"      test hook: the driver step with this number
+ *                         (1..4) fails once with STARSH_MALLOC_ERROR
+ *                         (0 clears);
  *   "release_workspace"   (value ignored) frees the device workspace and
  *                         the pinned staging buffer, which are otherwise
  *                         kept and reused between calls.

@@ -215,6 +215,22 @@ typedef int STARSH_blrm_approximate(STARSH_blrm **matrix, STARSH_blrf *format,
         int maxrank, double tol, int onfly);
 extern STARSH_blrm_approximate *starsh_blrm_approximate;
 
+/* ---- Error measure, product and solver under the reference's own names
+ * (include/starsh.h:601-604, 622-623, 671-672), so that reference callers
+ * such as testing/spatial.c:101 link unchanged.  In this library they run
+ * on the GPU: starsh_blrm__dfe and starsh_blrm__dfe_omp are
+ * starsh_blrm__dfe_hip (returns -1 on failure, as openmp/blrm/dfe.c:91-93);
+ * starsh_blrm__dmml and starsh_blrm__dmml_omp are starsh_blrm__dmml_hip;
+ * starsh_itersolvers__dcg_omp is starsh_itersolvers__dcg_hip. */
+double starsh_blrm__dfe(STARSH_blrm *matrix);
+double starsh_blrm__dfe_omp(STARSH_blrm *matrix);
+int starsh_blrm__dmml(STARSH_blrm *matrix, int nrhs, double alpha,
+        double *A, int lda, double beta, double *B, int ldb);
+int starsh_blrm__dmml_omp(STARSH_blrm *matrix, int nrhs, double alpha,
+        double *A, int lda, double beta, double *B, int ldb);
+int starsh_itersolvers__dcg_omp(STARSH_blrm *matrix, int nrhs, double *B,
+        int ldb, double *X, int ldx, double tol, double *work);
+
 /* ---- Dense helper (reference include/starsh.h:642,
  * src/backends/sequential/dense/dsvfr.c:16-47): rank from singular values
  * by relative Frobenius tail. */

@@ -55,6 +55,10 @@ int starsh_hip_set_option(const char *name, double value)
         return starsh_hip_qr_panel((int)value);
     if(!strcmp(name, "proj_int8"))
         return starsh_hip_proj_int8((int)value);
+    if(!strcmp(name, "jacobi_max_sweeps"))
+        return starsh_hip_jacobi_sweeps((int)value);
+    if(!strcmp(name, "inject_failure"))
+        return hip_driver_inject_failure((int)value);
     if(!strcmp(name, "release_workspace"))
     {
         rsdd_release_pools();
@@ -136,12 +140,34 @@ int hip_driver_prepare(STARSH_blrf *F, int maxrank, double tol, int onfly,
                 "are supported)");
         return STARSH_WRONG_PARAMETER;
     }
-    int info = starsh_hip_init(0);
-    if(info != STARSH_SUCCESS)
-        return info;
     hp->oversample = starsh_params.oversample > 0 ? starsh_params.oversample
         : 10;
     STARSH_cluster *RC = F->row_cluster, *CC = F->col_cluster;
+    /* Limits of the device pipeline, checked before any GPU work: tiles of
+     * at most HIP_TILE_MAX rows and columns (Gram-Schmidt panels keep a
+     * column in registers), sketch widths of at most HIP_SKETCH_MAX (core
+     * SVD in one workgroup). */
+    for(STARSH_int bi = 0; bi < F->nblocks_far; bi++)
+    {
+        const STARSH_int m = RC->size[F->block_far[2*bi]];
+        const STARSH_int n = CC->size[F->block_far[2*bi+1]];
+        const STARSH_int mn = m < n ? m : n;
+        const STARSH_int k = (STARSH_int)maxrank+hp->oversample < mn ?
+            (STARSH_int)maxrank+hp->oversample : mn;
+        if(m > HIP_TILE_MAX || n > HIP_TILE_MAX || k > HIP_SKETCH_MAX)
+        {
+            STARSH_ERROR("far-field tile %lld is %lld x %lld with sketch "
+                    "width %lld; the HIP backend takes tiles of at most %d "
+                    "rows and columns and sketch widths "
+                    "min(maxrank+oversample, m, n) of at most %d",
+                    (long long)bi, (long long)m, (long long)n, (long long)k,
+                    HIP_TILE_MAX, HIP_SKETCH_MAX);
+            return STARSH_WRONG_PARAMETER;
+        }
+    }
+    int info = starsh_hip_init(0);
+    if(info != STARSH_SUCCESS)
+        return info;
     info = upload_points(xr, cr, hp->kd.ndim, RC, &hp->d_xr);
     if(info != STARSH_SUCCESS)
         return info;
@@ -173,7 +199,8 @@ void hip_driver_release(hip_problem *hp)
     hp->d_xr = hp->d_xc = NULL;
 }
 
-/* Steps 1: host buffers and wrappers (drsdd.c:53-90). */
+/* Steps 1: host buffers and wrappers (drsdd.c:53-90).  hf owns what it
+ * points to; on failure hip_driver_free_factors releases it. */
 int hip_driver_alloc_factors(STARSH_blrf *F, int maxrank, hip_factors *hf)
 {
     memset(hf, 0, sizeof(*hf));
@@ -181,11 +208,14 @@ int hip_driver_alloc_factors(STARSH_blrf *F, int maxrank, hip_factors *hf)
     STARSH_cluster *RC = F->row_cluster, *CC = F->col_cluster;
     if(nfar == 0)
         return STARSH_SUCCESS;
-    STARSH_MALLOC(hf->far_U, nfar);
-    STARSH_MALLOC(hf->far_V, nfar);
+    hf->far_U = calloc(nfar, sizeof(Array *));
+    hf->far_V = calloc(nfar, sizeof(Array *));
+    hf->nfar = nfar;
     STARSH_MALLOC(hf->far_rank, nfar);
     STARSH_MALLOC(hf->uoff, nfar);
     STARSH_MALLOC(hf->voff, nfar);
+    if(hf->far_U == NULL || hf->far_V == NULL)
+        return STARSH_MALLOC_ERROR;
     size_t su = 0, sv = 0;
     for(STARSH_int bi = 0; bi < nfar; bi++)
     {
@@ -211,22 +241,23 @@ int hip_driver_alloc_factors(STARSH_blrf *F, int maxrank, hip_factors *hf)
     return STARSH_SUCCESS;
 }
 
+/* Releases what hf holds.  The wrappers only view alloc_U/alloc_V, so
+ * their data pointers are detached before array_free. */
 void hip_driver_free_factors(STARSH_blrf *F, hip_factors *hf)
 {
-    if(hf->far_U)
-        for(STARSH_int bi = 0; bi < F->nblocks_far; bi++)
-            if(hf->far_U[bi])
-            {
-                hf->far_U[bi]->data = NULL;
-                array_free(hf->far_U[bi]);
-            }
-    if(hf->far_V)
-        for(STARSH_int bi = 0; bi < F->nblocks_far; bi++)
-            if(hf->far_V[bi])
-            {
-                hf->far_V[bi]->data = NULL;
-                array_free(hf->far_V[bi]);
-            }
+    (void)F;
+    for(STARSH_int bi = 0; hf->far_U && bi < hf->nfar; bi++)
+        if(hf->far_U[bi])
+        {
+            hf->far_U[bi]->data = NULL;
+            array_free(hf->far_U[bi]);
+        }
+    for(STARSH_int bi = 0; hf->far_V && bi < hf->nfar; bi++)
+        if(hf->far_V[bi])
+        {
+            hf->far_V[bi]->data = NULL;
+            array_free(hf->far_V[bi]);
+        }
     free(hf->far_U);
     free(hf->far_V);
     free(hf->far_rank);
@@ -237,7 +268,44 @@ void hip_driver_free_factors(STARSH_blrf *F, hip_factors *hf)
     memset(hf, 0, sizeof(*hf));
 }
 
-/* Steps 3-5 (drsdd.c:146-306). */
+/* Releases near-field wrappers and their buffer after a failure. */
+static void free_near(Array **near_D, STARSH_int n, double *alloc_D)
+{
+    for(STARSH_int bi = 0; near_D && bi < n; bi++)
+        if(near_D[bi])
+        {
+            near_D[bi]->data = NULL;
+            array_free(near_D[bi]);
+        }
+    free(near_D);
+    free(alloc_D);
+}
+
+/* Fault injection for tests of the error paths: the next call of
+ * inject_failure() with this site returns nonzero once. */
+static int injected_site = 0;
+
+int hip_driver_inject_failure(int site)
+{
+    if(site < 0)
+        return STARSH_WRONG_PARAMETER;
+    injected_site = site;
+    return STARSH_SUCCESS;
+}
+
+static int inject_failure(int site)
+{
+    if(injected_site != 0 && injected_site == site)
+    {
+        injected_site = 0;
+        return 1;
+    }
+    return 0;
+}
+
+/* Steps 3-5 (drsdd.c:146-306).  After each step hf describes exactly the
+ * far-field arrays still owned by the driver, so that a failure at any
+ * point can be cleaned up by hip_driver_free_factors. */
 int hip_driver_finish(STARSH_blrm **matrix, STARSH_blrf *format,
         const hip_problem *hp, hip_factors *hf, int onfly,
         starsh_hip_stats *st)
@@ -247,54 +315,60 @@ int hip_driver_finish(STARSH_blrm **matrix, STARSH_blrf *format,
     STARSH_cluster *RC = F->row_cluster, *CC = F->col_cluster;
     STARSH_int nblocks_far = F->nblocks_far, nblocks_near = F->nblocks_near;
     STARSH_int new_far = nblocks_far, new_near = nblocks_near;
-    STARSH_int *block_far = F->block_far, *block_near = F->block_near;
-    int *far_rank = hf->far_rank;
-    Array **far_U = hf->far_U, **far_V = hf->far_V, **near_D = NULL;
-    double *alloc_U = hf->alloc_U, *alloc_V = hf->alloc_V, *alloc_D = NULL;
+    STARSH_int *block_near = F->block_near;
     STARSH_int nfalse = 0, bi, bj;
     for(bi = 0; bi < nblocks_far; bi++)
-        if(far_rank[bi] == -1)
+        if(hf->far_rank[bi] == -1)
             nfalse++;
     st->nfalse_far += nfalse;
     if(nfalse > 0)
     {
-        STARSH_int *false_far;
-        STARSH_MALLOC(false_far, nfalse);
-        for(bi = 0, bj = 0; bi < nblocks_far; bi++)
-            if(far_rank[bi] == -1)
-                false_far[bj++] = bi;
+        STARSH_int *false_far = malloc(sizeof(STARSH_int)*nfalse);
+        STARSH_int *bnear = NULL, *bfar = NULL;
         new_near = nblocks_near+nfalse;
-        STARSH_MALLOC(block_near, 2*new_near);
-        for(bi = 0; bi < 2*nblocks_near; bi++)
-            block_near[bi] = F->block_near[bi];
-        for(bi = 0; bi < nfalse; bi++)
-        {
-            block_near[2*(bi+nblocks_near)] = F->block_far[2*false_far[bi]];
-            block_near[2*(bi+nblocks_near)+1] =
-                F->block_far[2*false_far[bi]+1];
-        }
         new_far = nblocks_far-nfalse;
-        block_far = NULL;
+        bnear = malloc(sizeof(STARSH_int)*2*new_near);
         if(new_far > 0)
-        {
-            STARSH_MALLOC(block_far, 2*new_far);
-            for(bi = 0, bj = 0; bi < nblocks_far; bi++)
-            {
-                if(bj < nfalse && false_far[bj] == bi)
-                    bj++;
-                else
-                {
-                    block_far[2*(bi-bj)] = F->block_far[2*bi];
-                    block_far[2*(bi-bj)+1] = F->block_far[2*bi+1];
-                }
-            }
-        }
-        STARSH_blrf *F2;
-        int info = starsh_blrf_new_from_coo(&F2, P, F->symm, RC, CC, new_far,
-                block_far, new_near, block_near, F->type);
-        if(info != STARSH_SUCCESS)
+            bfar = malloc(sizeof(STARSH_int)*2*new_far);
+        if(false_far == NULL || bnear == NULL || (new_far > 0 &&
+                    bfar == NULL) || inject_failure(1))
         {
             free(false_far);
+            free(bnear);
+            free(bfar);
+            STARSH_ERROR("malloc() failed");
+            return STARSH_MALLOC_ERROR;
+        }
+        for(bi = 0, bj = 0; bi < nblocks_far; bi++)
+            if(hf->far_rank[bi] == -1)
+                false_far[bj++] = bi;
+        for(bi = 0; bi < 2*nblocks_near; bi++)
+            bnear[bi] = F->block_near[bi];
+        for(bi = 0; bi < nfalse; bi++)
+        {
+            bnear[2*(bi+nblocks_near)] = F->block_far[2*false_far[bi]];
+            bnear[2*(bi+nblocks_near)+1] = F->block_far[2*false_far[bi]+1];
+        }
+        for(bi = 0, bj = 0; bi < nblocks_far; bi++)
+        {
+            if(bj < nfalse && false_far[bj] == bi)
+                bj++;
+            else
+            {
+                bfar[2*(bi-bj)] = F->block_far[2*bi];
+                bfar[2*(bi-bj)+1] = F->block_far[2*bi+1];
+            }
+        }
+        free(false_far);
+        /* on success starsh_blrf_new_from_coo owns both lists */
+        STARSH_blrf *F2;
+        int info = inject_failure(4) ? STARSH_MALLOC_ERROR :
+            starsh_blrf_new_from_coo(&F2, P, F->symm, RC, CC, new_far, bfar,
+                    new_near, bnear, F->type);
+        if(info != STARSH_SUCCESS)
+        {
+            free(bnear);
+            free(bfar);
             return info;
         }
         STARSH_blrf tmp = *F;
@@ -302,65 +376,61 @@ int hip_driver_finish(STARSH_blrm **matrix, STARSH_blrf *format,
         *F2 = tmp;
         STARSH_WARNING("`F` was modified due to false far-field blocks");
         starsh_blrf_free(F2);
-        /* Compaction of the far-field arrays (drsdd.c:259-282); the old
+        block_near = F->block_near;
+        /* Compaction of the far-field arrays (drsdd.c:259-282); the
          * wrappers of false far-field tiles are released. */
         for(bi = 0, bj = 0; bi < nblocks_far; bi++)
         {
-            if(far_rank[bi] == -1)
+            Array *u = hf->far_U[bi], *v = hf->far_V[bi];
+            hf->far_U[bi] = hf->far_V[bi] = NULL;
+            void *du = u->data, *dv = v->data;
+            int shape_U[2] = {u->shape[0], hf->far_rank[bi]};
+            int shape_V[2] = {v->shape[0], hf->far_rank[bi]};
+            u->data = NULL;
+            array_free(u);
+            v->data = NULL;
+            array_free(v);
+            if(hf->far_rank[bi] == -1)
             {
-                far_U[bi]->data = NULL;
-                array_free(far_U[bi]);
-                far_V[bi]->data = NULL;
-                array_free(far_V[bi]);
                 bj++;
                 continue;
             }
-            int shape_U[2] = {far_U[bi]->shape[0], far_rank[bi]};
-            int shape_V[2] = {far_V[bi]->shape[0], far_rank[bi]};
-            void *du = far_U[bi]->data, *dv = far_V[bi]->data;
-            far_U[bi]->data = NULL;
-            array_free(far_U[bi]);
-            far_V[bi]->data = NULL;
-            array_free(far_V[bi]);
-            array_from_buffer(far_U+bi-bj, 2, shape_U, 'd', 'F', du);
-            array_from_buffer(far_V+bi-bj, 2, shape_V, 'd', 'F', dv);
-            far_rank[bi-bj] = far_rank[bi];
+            info = array_from_buffer(hf->far_U+bi-bj, 2, shape_U, 'd', 'F',
+                    du);
+            if(info == STARSH_SUCCESS)
+                info = array_from_buffer(hf->far_V+bi-bj, 2, shape_V, 'd',
+                        'F', dv);
+            hf->far_rank[bi-bj] = hf->far_rank[bi];
+            if(info != STARSH_SUCCESS)
+            {
+                hf->nfar = bi-bj+1;
+                return info;
+            }
         }
-        free(false_far);
-        if(new_far > 0)
+        hf->nfar = new_far;
+        if(new_far == 0)
         {
-            STARSH_REALLOC(far_rank, new_far);
-            STARSH_REALLOC(far_U, new_far);
-            STARSH_REALLOC(far_V, new_far);
-        }
-        else
-        {
-            free(far_rank);
-            free(far_U);
-            free(far_V);
-            free(alloc_U);
-            free(alloc_V);
-            far_rank = NULL;
-            far_U = far_V = NULL;
-            alloc_U = alloc_V = NULL;
+            free(hf->alloc_U);
+            free(hf->alloc_V);
+            hf->alloc_U = hf->alloc_V = NULL;
         }
     }
     /* Near-field tiles (drsdd.c:214-256), generated on the GPU. */
+    Array **near_D = NULL;
+    double *alloc_D = NULL;
     if(onfly == 0 && new_near > 0)
     {
-        STARSH_MALLOC(near_D, new_near);
+        near_D = calloc(new_near, sizeof(Array *));
         int64_t *r0 = malloc(sizeof(int64_t)*new_near);
         int64_t *c0 = malloc(sizeof(int64_t)*new_near);
         int64_t *off = malloc(sizeof(int64_t)*new_near);
         int *mm = malloc(sizeof(int)*new_near), *nn = malloc(sizeof(int)*
                 new_near);
-        if(!r0 || !c0 || !off || !mm || !nn)
-        {
-            free(r0); free(c0); free(off); free(mm); free(nn);
-            return STARSH_MALLOC_ERROR;
-        }
+        int info = STARSH_SUCCESS;
         size_t size_D = 0;
-        for(bi = 0; bi < new_near; bi++)
+        if(!near_D || !r0 || !c0 || !off || !mm || !nn || inject_failure(2))
+            info = STARSH_MALLOC_ERROR;
+        for(bi = 0; info == STARSH_SUCCESS && bi < new_near; bi++)
         {
             STARSH_int i = block_near[2*bi], j = block_near[2*bi+1];
             r0[bi] = RC->start[i];
@@ -370,20 +440,24 @@ int hip_driver_finish(STARSH_blrm **matrix, STARSH_blrf *format,
             off[bi] = size_D;
             size_D += (size_t)mm[bi]*nn[bi];
         }
-        int info = STARSH_SUCCESS;
-        alloc_D = malloc(sizeof(double)*size_D + 1);
         double *d_D = NULL;
-        if(alloc_D == NULL)
-            info = STARSH_MALLOC_ERROR;
         if(info == STARSH_SUCCESS)
-            info = starsh_hip_malloc((void **)&d_D, sizeof(double)*size_D);
+        {
+            alloc_D = malloc(sizeof(double)*size_D + 1);
+            if(alloc_D == NULL)
+                info = STARSH_MALLOC_ERROR;
+        }
+        if(info == STARSH_SUCCESS)
+            info = inject_failure(3) ? STARSH_MALLOC_ERROR :
+                starsh_hip_malloc((void **)&d_D, sizeof(double)*size_D);
         if(info == STARSH_SUCCESS)
             info = starsh_hip_generate_tiles(&hp->kd, hp->d_xr, hp->nr,
                     hp->d_xc, hp->nc, (int)new_near, r0, c0, mm, nn, off,
                     d_D);
         if(info == STARSH_SUCCESS)
             info = starsh_hip_memcpy_d2h(alloc_D, d_D, sizeof(double)*size_D);
-        starsh_hip_free(d_D);
+        if(d_D != NULL)
+            starsh_hip_free(d_D);
         for(bi = 0; bi < new_near && info == STARSH_SUCCESS; bi++)
         {
             int shape[2] = {mm[bi], nn[bi]};
@@ -392,15 +466,51 @@ int hip_driver_finish(STARSH_blrm **matrix, STARSH_blrf *format,
         }
         free(r0); free(c0); free(off); free(mm); free(nn);
         if(info != STARSH_SUCCESS)
+        {
+            STARSH_ERROR("near-field tiles failed");
+            free_near(near_D, new_near, alloc_D);
             return info;
+        }
     }
-    hf->far_U = NULL;
-    hf->far_V = NULL;
-    hf->far_rank = NULL;
-    hf->alloc_U = NULL;
-    hf->alloc_V = NULL;
-    return starsh_blrm_new(matrix, F, far_rank, far_U, far_V, onfly, near_D,
-            alloc_U, alloc_V, alloc_D, '1');
+    int *far_rank = hf->far_rank;
+    Array **far_U = hf->far_U, **far_V = hf->far_V;
+    double *alloc_U = hf->alloc_U, *alloc_V = hf->alloc_V;
+    if(new_far == 0)
+    {
+        free(far_rank);
+        free(far_U);
+        free(far_V);
+        far_rank = NULL;
+        far_U = far_V = NULL;
+    }
+    else
+    {
+        /* shrink the pointer arrays; a failed shrink keeps the old ones */
+        void *q;
+        if((q = realloc(far_rank, sizeof(int)*new_far)) != NULL)
+            far_rank = q;
+        if((q = realloc(far_U, sizeof(Array *)*new_far)) != NULL)
+            far_U = q;
+        if((q = realloc(far_V, sizeof(Array *)*new_far)) != NULL)
+            far_V = q;
+    }
+    free(hf->uoff);
+    free(hf->voff);
+    memset(hf, 0, sizeof(*hf));
+    int info = starsh_blrm_new(matrix, F, far_rank, far_U, far_V, onfly,
+            near_D, alloc_U, alloc_V, alloc_D, '1');
+    if(info != STARSH_SUCCESS)
+    {
+        /* give the buffers back to hf so the caller's cleanup frees them */
+        hf->far_rank = far_rank;
+        hf->far_U = far_U;
+        hf->far_V = far_V;
+        hf->alloc_U = alloc_U;
+        hf->alloc_V = alloc_V;
+        hf->nfar = new_far;
+        free_near(near_D, onfly == 0 ? new_near : 0, alloc_D);
+    }
+    return info;
 }
 
 /* Bytes of the pipeline's workspace for one tile. */
@@ -508,7 +618,9 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
         out.flags = flags_out+b0;
         out.uoff = uo+b0;
         out.voff = vo+b0;
+        double tb0 = hip_driver_now();
         info = rsdd_batch(&in, &out);
+        st->batch_seconds += hip_driver_now()-tb0;
         if(info != STARSH_SUCCESS)
             break;
         st->device_seconds += out.device_seconds;
@@ -536,7 +648,7 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
             dv[t] = hf->alloc_V+hf->voff[bi];
         }
         info = rsdd_copy_factors(&out, (int)(b1-b0), m+b0, n+b0, du+b0,
-                dv+b0);
+                dv+b0, &st->d2h_seconds, &st->scatter_seconds);
         st->copy_seconds += hip_driver_now()-tc;
         b0 = b1;
     }
@@ -629,6 +741,7 @@ int hip_compress(int mixed, int nprob, STARSH_blrm **matrix,
         nprep++;
         info = hip_driver_alloc_factors(format[q], maxrank[q], B.hf+q);
     }
+    st.prepare_seconds = hip_driver_now()-t_start;
     /* all far-field tiles, grouped by kernel family and dimension */
     STARSH_int ntot = 0;
     for(int q = 0; info == STARSH_SUCCESS && q < nprob; q++)
@@ -660,9 +773,11 @@ int hip_compress(int mixed, int nprob, STARSH_blrm **matrix,
     free(pidx); free(idx); free(done);
     if(info == STARSH_SUCCESS)
         hip_driver_rank_stats(&B, &st);
+    double tf = hip_driver_now();
     for(int q = 0; info == STARSH_SUCCESS && q < nprob; q++)
         info = hip_driver_finish(matrix+q, format[q], B.hp+q, B.hf+q, onfly,
                 &st);
+    st.finish_seconds = hip_driver_now()-tf;
     if(info != STARSH_SUCCESS)
         for(int q = 0; q < nprob; q++)
         {

@@ -146,7 +146,9 @@ static int sched_first(const sched_entry *key)
  * when p >= max(oversample, r), i.e. the factor stays below sqrt(2). */
 static int accept_mixed(int rank, int k, int kfull, int flags, int os)
 {
-    if(k >= kfull)
+    /* flag 4: the core SVD did not converge; the tile is a false far-field
+     * tile (rank -1), as when dgesdd fails in drsdd.c:78 */
+    if(k >= kfull || (flags & 4) != 0)
         return 1;
     if(rank < 0)
         return 0;

@@ -8,14 +8,15 @@
 static int group_by(STARSH_int nb, STARSH_int n, const STARSH_int *coo, int k,
         STARSH_int **start_out, STARSH_int **list_out)
 {
-    STARSH_int *start, *list, *fill;
-    STARSH_MALLOC(start, nb+1);
-    STARSH_MALLOC(list, n);
-    fill = calloc(nb+1, sizeof(*fill));
-    if(fill == NULL)
+    STARSH_int *start = malloc(sizeof(*start)*(nb+1));
+    STARSH_int *list = malloc(sizeof(*list)*n+1);
+    STARSH_int *fill = calloc(nb+1, sizeof(*fill));
+    if(start == NULL || list == NULL || fill == NULL)
     {
         free(start);
         free(list);
+        free(fill);
+        STARSH_ERROR("malloc() failed");
         return STARSH_MALLOC_ERROR;
     }
     for(STARSH_int b = 0; b < n; b++)
@@ -75,26 +76,40 @@ int starsh_blrf_new_from_coo(STARSH_blrf **format, STARSH_problem *problem,
     F->nblocks_near_local = nblocks_near;
     F->block_near_local = F->block_near;
     F->type = type;
-    int info;
-    if(nblocks_far > 0 && (info = group_by(F->nbrows, nblocks_far, block_far,
-                    0, &F->brow_far_start, &F->brow_far)) != 0)
-        return info;
-    if(nblocks_near > 0 && (info = group_by(F->nbrows, nblocks_near,
-                    block_near, 0, &F->brow_near_start, &F->brow_near)) != 0)
-        return info;
-    if(symm == 'N')
+    int info = STARSH_SUCCESS;
+    if(nblocks_far > 0)
+        info = group_by(F->nbrows, nblocks_far, block_far, 0,
+                &F->brow_far_start, &F->brow_far);
+    if(info == STARSH_SUCCESS && nblocks_near > 0)
+        info = group_by(F->nbrows, nblocks_near, block_near, 0,
+                &F->brow_near_start, &F->brow_near);
+    if(info == STARSH_SUCCESS && symm == 'N')
     {
         F->col_cluster = col_cluster;
         F->nbcols = col_cluster->nblocks;
-        if(nblocks_far > 0 && (info = group_by(F->nbcols, nblocks_far,
-                        block_far, 1, &F->bcol_far_start, &F->bcol_far)) != 0)
-            return info;
-        if(nblocks_near > 0 && (info = group_by(F->nbcols, nblocks_near,
-                        block_near, 1, &F->bcol_near_start,
-                        &F->bcol_near)) != 0)
-            return info;
+        if(nblocks_far > 0)
+            info = group_by(F->nbcols, nblocks_far, block_far, 1,
+                    &F->bcol_far_start, &F->bcol_far);
+        if(info == STARSH_SUCCESS && nblocks_near > 0)
+            info = group_by(F->nbcols, nblocks_near, block_near, 1,
+                    &F->bcol_near_start, &F->bcol_near);
     }
-    else
+    if(info != STARSH_SUCCESS)
+    {
+        /* the caller keeps the COO lists on failure; everything built
+         * here is released */
+        free(F->brow_far_start);
+        free(F->brow_far);
+        free(F->brow_near_start);
+        free(F->brow_near);
+        free(F->bcol_far_start);
+        free(F->bcol_far);
+        free(F->bcol_near_start);
+        free(F->bcol_near);
+        free(F);
+        return info;
+    }
+    if(symm == 'S')
     {
         F->col_cluster = row_cluster;
         F->nbcols = row_cluster->nblocks;
@@ -146,8 +161,11 @@ int starsh_blrf_new_tlr(STARSH_blrf **format, STARSH_problem *problem,
             k++;
         }
     }
-    return starsh_blrf_new_from_coo(format, problem, symm, row_cluster,
+    int info = starsh_blrf_new_from_coo(format, problem, symm, row_cluster,
             col_cluster, nfar, block_far, 0, NULL, STARSH_TLR);
+    if(info != STARSH_SUCCESS)
+        free(block_far);
+    return info;
 }
 
 void starsh_blrf_free(STARSH_blrf *format)

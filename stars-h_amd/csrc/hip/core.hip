@@ -13,41 +13,18 @@
 // singular values; Jacobi on R2^T converges in fewer sweeps than on R2.
 #include <float.h>
 #include "pipeline.h"
+#include "jacobi_rotation.h"
 
 namespace {
 
 constexpr int CORE_THREADS = 256;     // tiles whose W fits in LDS
 constexpr int CORE_THREADS_BIG = 1024; // W in global memory
 constexpr int LDS_PMAX = 128;
+constexpr int JACOBI_SWEEPS_DEFAULT = 60;
 
 __device__ inline double group_sum(double v, int g)
 {
     return group_sum_lanes(v, g);
-}
-
-// Rotation (c, s) that makes two columns with squared norms al, be and
-// inner product ga orthogonal: Rutishauser's t = tan(angle) =
-// sign(zeta)/(|zeta| + sqrt(1 + zeta^2)), zeta = (be - al)/(2 ga), the
-// smaller root of t^2 + 2 zeta t - 1 = 0, written with d = be - al and
-// h = 2 ga as t = sign(d) h/(|d| + sqrt(d^2 + h^2)), so that one division
-// and one square root do; c = 1/sqrt(1 + t^2) from the reciprocal square
-// root.  Operands beyond 1e150 (whose squares would overflow) take the
-// zeta form.
-__device__ inline void jacobi_rotation(double al, double be, double ga,
-        double &c, double &s)
-{
-    const double d = be-al, h = 2.0*ga;
-    double t;
-    if(fabs(d) < 1e150 && fabs(h) < 1e150)
-        t = (d >= 0.0 ? h : -h)/(fabs(d)+sqrt(fma(d, d, h*h)));
-    else
-    {
-        const double zeta = d/h;
-        t = fabs(zeta) > 1e150 ? 0.5/zeta : (zeta >= 0.0 ? 1.0 : -1.0)/
-            (fabs(zeta)+sqrt(1.0+zeta*zeta));
-    }
-    c = rsqrt(fma(t, t, 1.0));
-    s = c*t;
 }
 
 // One-sided Jacobi step on the column pair (wa, wb) of length p by a group
@@ -66,7 +43,7 @@ __device__ inline bool rotate_pair(double *wa, double *wb, int p, int g,
     al = group_sum(al, g);
     be = group_sum(be, g);
     ga = group_sum(ga, g);
-    if(!(al > 0.0 && be > 0.0 && fabs(ga) > ctol*sqrt(al*be)))
+    if(!jr_rotates(al, be, ga, ctol))
         return false;
     double c, s;
     jacobi_rotation(al, be, ga, c, s);
@@ -106,7 +83,7 @@ __device__ inline bool rotate_pair_reg(double *wa, double *wb, int p, int g,
     al = group_sum(al, g);
     be = group_sum(be, g);
     ga = group_sum(ga, g);
-    if(!(al > 0.0 && be > 0.0 && fabs(ga) > ctol*sqrt(al*be)))
+    if(!jr_rotates(al, be, ga, ctol))
         return false;
     double c, s;
     jacobi_rotation(al, be, ga, c, s);
@@ -151,7 +128,7 @@ __device__ inline bool rotate_reg(double (&ca)[CREG], double *wb, int p,
     al = group_sum(al, 64);
     be = group_sum(be, 64);
     ga = group_sum(ga, 64);
-    if(!(al > 0.0 && be > 0.0 && fabs(ga) > ctol*sqrt(al*be)))
+    if(!jr_rotates(al, be, ga, ctol))
         return false;
     double c, s;
     jacobi_rotation(al, be, ga, c, s);
@@ -176,7 +153,7 @@ __device__ inline bool rotate_reg(double (&ca)[CREG], double *wb, int p,
 // access of a column is one coalesced 512-byte segment.
 template <int NT>
 __device__ void jacobi(double *W, int ldw, int p, bool in_lds,
-        int *sweeps_out)
+        int max_sweeps, int *sweeps_out)
 {
     __shared__ int rotations;
     const int pe = p+(p & 1);
@@ -189,7 +166,7 @@ __device__ void jacobi(double *W, int ldw, int p, bool in_lds,
     const int grp = threadIdx.x/g, gl = threadIdx.x % g;
     const double ctol = (double)p*DBL_EPSILON;
     int sweep;
-    for(sweep = 0; sweep < 60; sweep++)
+    for(sweep = 0; sweep < max_sweeps; sweep++)
     {
         if(threadIdx.x == 0)
             rotations = 0;
@@ -235,7 +212,7 @@ __device__ void jacobi(double *W, int ldw, int p, bool in_lds,
 // (L: 2 BW columns of p doubles), so each block pair costs one load and
 // one store of its columns instead of one pass over W per round.
 __device__ void jacobi_blocked(double *W, int p, double *L, int bw,
-        int *sweeps_out)
+        int max_sweeps, int *sweeps_out)
 {
     __shared__ int rotations;
     const int nb = (p+bw-1)/bw;
@@ -256,7 +233,7 @@ __device__ void jacobi_blocked(double *W, int p, double *L, int bw,
             dst[e] = src[e];
     };
     int sweep;
-    for(sweep = 0; sweep < 60; sweep++)
+    for(sweep = 0; sweep < max_sweeps; sweep++)
     {
         if(threadIdx.x == 0)
             rotations = 0;
@@ -360,7 +337,7 @@ __device__ inline bool rotate_reg32(double (&ca)[CR32], double *wb, int p,
     al = group_sum(al, 32);
     be = group_sum(be, 32);
     ga = group_sum(ga, 32);
-    if(!(al > 0.0 && be > 0.0 && fabs(ga) > ctol*sqrt(al*be)))
+    if(!jr_rotates(al, be, ga, ctol))
         return false;
     double c, s;
     jacobi_rotation(al, be, ga, c, s);
@@ -384,7 +361,8 @@ __device__ inline bool rotate_reg32(double (&ca)[CR32], double *wb, int p,
 // half-wave a takes its column a into registers, and the later blocks J
 // pass through L one after the other; column a goes back to W at the end.
 // The same pairs in the same order as jacobi_blocked with bw = 32.
-__device__ void jacobi_blocked32(double *W, int p, double *L, int *sweeps_out)
+__device__ void jacobi_blocked32(double *W, int p, double *L, int max_sweeps,
+        int *sweeps_out)
 {
     constexpr int BW = 32;
     __shared__ int rotations;
@@ -406,7 +384,7 @@ __device__ void jacobi_blocked32(double *W, int p, double *L, int *sweeps_out)
             dst[e] = L[e];
     };
     int sweep;
-    for(sweep = 0; sweep < 60; sweep++)
+    for(sweep = 0; sweep < max_sweeps; sweep++)
     {
         if(threadIdx.x == 0)
             rotations = 0;
@@ -489,7 +467,7 @@ __device__ void jacobi_blocked32(double *W, int p, double *L, int *sweeps_out)
 // (W in their workspace, blocked Jacobi through LDS).
 template <int NT>
 __global__ __launch_bounds__(NT) void core_kernel(TileDesc *tiles,
-        double *ws, int pmax, int lds_p, int big, int bw)
+        double *ws, int pmax, int lds_p, int big, int bw, int max_sweeps)
 {
     extern __shared__ double dyn[];
     double *sig = dyn;
@@ -498,6 +476,7 @@ __global__ __launch_bounds__(NT) void core_kernel(TileDesc *tiles,
     int *perm = (int *)(lds+(big ? (int64_t)(bw == 32 ? 1 : 2)*bw*pmax :
                 (int64_t)lds_p*lds_p));
     __shared__ int s_rank;
+    __shared__ unsigned long long s_wmax;
     TileDesc &td = tiles[blockIdx.x];
     const int p = td.p, k = td.k, maxrank = td.maxrank;
     const double tol = td.tol;
@@ -525,12 +504,36 @@ __global__ __launch_bounds__(NT) void core_kernel(TileDesc *tiles,
         W = ws+td.r_off+(int64_t)k*k;
         ldw = p;
     }
-    // W = R2^T
+    // W = R2^T, scaled by the power of two 2^-e that brings its largest
+    // entry into [1/2, 1).  The scaling is exact, so the rotations are
+    // those of R2^T itself, but squared norms and their squares inside the
+    // rotation stay far from the bottom of the FP64 range for tiles whose
+    // entries are tiny (far tiles of fast-decaying kernels).
+    if(threadIdx.x == 0)
+        s_wmax = 0;
+    __syncthreads();
+    double wmax = 0.0;
     for(int e = threadIdx.x; e < p*p; e += blockDim.x)
     {
         int i = e % p, j = e / p;
-        W[(int64_t)j*ldw+i] = R2[(int64_t)i*k+j];
+        const double v = R2[(int64_t)i*k+j];
+        W[(int64_t)j*ldw+i] = v;
+        wmax = fmax(wmax, fabs(v));
     }
+    // non-negative doubles order like their bit patterns
+    atomicMax(&s_wmax, (unsigned long long)__double_as_longlong(wmax));
+    __syncthreads();
+    int wexp = 0;
+    const double wm = __longlong_as_double((long long)s_wmax);
+    if(wm > 0.0 && wm < INFINITY)
+        frexp(wm, &wexp);
+    const double wscale = ldexp(1.0, -wexp), wunscale = ldexp(1.0, wexp);
+    if(wexp != 0)
+        for(int e = threadIdx.x; e < p*p; e += blockDim.x)
+        {
+            int i = e % p, j = e / p;
+            W[(int64_t)j*ldw+i] *= wscale;
+        }
     __syncthreads();
     int sweeps = 0;
     if(big)
@@ -541,13 +544,13 @@ __global__ __launch_bounds__(NT) void core_kernel(TileDesc *tiles,
         if constexpr(NT == CORE_THREADS_BIG)
         {
             if(bw == 32)
-                jacobi_blocked32(W, p, lds, &sweeps);
+                jacobi_blocked32(W, p, lds, max_sweeps, &sweeps);
             else
-                jacobi_blocked(W, p, lds, bw, &sweeps);
+                jacobi_blocked(W, p, lds, bw, max_sweeps, &sweeps);
         }
     }
     else
-        jacobi<NT>(W, ldw, p, true, &sweeps);
+        jacobi<NT>(W, ldw, p, true, max_sweeps, &sweeps);
     __syncthreads();
     // Singular values: column norms of W.
     for(int j = threadIdx.x; j < p; j += blockDim.x)
@@ -573,7 +576,7 @@ __global__ __launch_bounds__(NT) void core_kernel(TileDesc *tiles,
     {
         double v = 0.0;
         if(j < p)
-            v = sig[perm[j]];
+            v = sig[perm[j]]*wunscale;
         else if(j == p)
             v = sqrt(td.tail2);
         S[j] = v;
@@ -603,11 +606,15 @@ __global__ __launch_bounds__(NT) void core_kernel(TileDesc *tiles,
         }
         if(r > p)
             r = p;
-        if(r > maxrank)
+        // a rank above maxrank, or a core SVD that did not converge within
+        // the sweep limit (the counterpart of info != 0 from dgesdd), marks
+        // a false far-field tile (drsdd.c:78,89-92)
+        const bool unconverged = sweeps >= max_sweeps;
+        if(r > maxrank || unconverged)
             r = -1;
         s_rank = r;
         td.rank = r;
-        td.flags |= sweeps >= 60 ? 4 : 0;
+        td.flags |= unconverged ? 4 : 0;
     }
     __syncthreads();
     const int r = s_rank;
@@ -624,6 +631,18 @@ __global__ __launch_bounds__(NT) void core_kernel(TileDesc *tiles,
 
 } // namespace
 
+// Sweep limit of the Jacobi core: a tile whose core has not converged
+// after this many sweeps is reported as a false far-field tile.
+static int jacobi_max_sweeps = JACOBI_SWEEPS_DEFAULT;
+
+extern "C" int starsh_hip_jacobi_sweeps(int sweeps)
+{
+    if(sweeps < 0 || sweeps > JACOBI_SWEEPS_DEFAULT)
+        return STARSH_WRONG_PARAMETER;
+    jacobi_max_sweeps = sweeps == 0 ? JACOBI_SWEEPS_DEFAULT : sweeps;
+    return STARSH_SUCCESS;
+}
+
 int pipeline_core(TileDesc *d_tiles, int ntiles, double *ws, int pmax,
         hipStream_t s)
 {
@@ -638,7 +657,8 @@ int pipeline_core(TileDesc *d_tiles, int ntiles, double *ws, int pmax,
     HIP_TRY(hipFuncSetAttribute((const void *)core_kernel<CORE_THREADS>,
                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     hipLaunchKernelGGL(core_kernel<CORE_THREADS>, dim3(ntiles),
-            dim3(CORE_THREADS), bytes, s, d_tiles, ws, pm, lds_p, 0, 0);
+            dim3(CORE_THREADS), bytes, s, d_tiles, ws, pm, lds_p, 0, 0,
+            jacobi_max_sweeps);
     HIP_TRY(hipGetLastError());
     if(pm > LDS_PMAX)
     {
@@ -652,7 +672,8 @@ int pipeline_core(TileDesc *d_tiles, int ntiles, double *ws, int pmax,
                     (const void *)core_kernel<CORE_THREADS_BIG>,
                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bb));
         hipLaunchKernelGGL(core_kernel<CORE_THREADS_BIG>, dim3(ntiles),
-                dim3(CORE_THREADS_BIG), bb, s, d_tiles, ws, pm, 0, 1, bw);
+                dim3(CORE_THREADS_BIG), bb, s, d_tiles, ws, pm, 0, 1, bw,
+                jacobi_max_sweeps);
         HIP_TRY(hipGetLastError());
     }
     return STARSH_SUCCESS;

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <map>
 #include <thread>
+#include <chrono>
 #include <vector>
 #include <utility>
 #include "pipeline.h"
@@ -398,7 +399,18 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     for(const auto &kp : kps)
         g.noisy |= kp.noise != 0.0;
 
-    hipEvent_t ev[4];
+    // destroyed on every return path
+    struct Events
+    {
+        hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+        ~Events()
+        {
+            for(auto x : e)
+                if(x)
+                    (void)hipEventDestroy(x);
+        }
+    } evs;
+    hipEvent_t *ev = evs.e;
     for(int e = 0; e < 4; e++)
         HIP_TRY(hipEventCreate(&ev[e]));
     HIP_TRY(hipEventRecord(ev[0], s));
@@ -485,8 +497,6 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     float ms = 0, ms_sk = 0;
     HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
     HIP_TRY(hipEventElapsedTime(&ms_sk, ev[2], ev[3]));
-    for(int e = 0; e < 4; e++)
-        (void)hipEventDestroy(ev[e]);
     out->device_seconds = ms*1e-3;
     out->sketch_seconds = ms_sk*1e-3;
     for(int t = 0; t < nt; t++)
@@ -506,8 +516,10 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
 // segments are packed on the device, brought over in one copy through
 // pinned memory, and scattered on the host by several threads.
 int rsdd_copy_factors(const RsddOutput *out, int ntiles, const int *m,
-        const int *n, double *const *dst_u, double *const *dst_v)
+        const int *n, double *const *dst_u, double *const *dst_v,
+        double *d2h_seconds, double *scatter_seconds)
 {
+    const auto t0 = std::chrono::steady_clock::now();
     hipStream_t s = starsh_hip_stream();
     std::vector<int64_t> seg;
     std::vector<double *> dst;
@@ -545,6 +557,7 @@ int rsdd_copy_factors(const RsddOutput *out, int ntiles, const int *m,
     HIP_TRY(hipMemcpyAsync(h, d_pack, sizeof(double)*total,
                 hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    const auto t1 = std::chrono::steady_clock::now();
     // host scatter: segments split over threads by size
     unsigned nth = std::thread::hardware_concurrency();
     nth = nth < 1 ? 1 : (nth > 16 ? 16 : nth);
@@ -566,6 +579,9 @@ int rsdd_copy_factors(const RsddOutput *out, int ntiles, const int *m,
     work(0);
     for(auto &x : th)
         x.join();
+    const auto t2 = std::chrono::steady_clock::now();
+    *d2h_seconds += std::chrono::duration<double>(t1-t0).count();
+    *scatter_seconds += std::chrono::duration<double>(t2-t1).count();
     return STARSH_SUCCESS;
 }
 

@@ -5,6 +5,10 @@
 #include "starsh.h"
 #include "starsh-hip.h"
 
+/* Largest tile side and sketch width the device pipeline takes. */
+#define HIP_TILE_MAX 4096
+#define HIP_SKETCH_MAX 1024
+
 typedef struct hip_problem
 {
     starsh_hip_kernel_desc kd;
@@ -19,6 +23,7 @@ typedef struct hip_factors
     int *far_rank;
     double *alloc_U, *alloc_V;
     size_t *uoff, *voff;   /* offsets of each tile in alloc_U / alloc_V */
+    STARSH_int nfar;       /* entries of far_U, far_V and far_rank */
 } hip_factors;
 
 int hip_driver_prepare(STARSH_blrf *F, int maxrank, double tol, int onfly,
@@ -30,6 +35,10 @@ int hip_driver_finish(STARSH_blrm **matrix, STARSH_blrf *format,
         const hip_problem *hp, hip_factors *hf, int onfly,
         starsh_hip_stats *st);
 void hip_driver_set_stats(const starsh_hip_stats *stats);
+/* Makes the driver step numbered `site` fail once with STARSH_MALLOC_ERROR
+ * (1: false far-field lists, 2: near-field lists, 3: near-field device
+ * buffer, 4: format rebuild); 0 clears.  For tests of the error paths. */
+int hip_driver_inject_failure(int site);
 
 /* Several problems compressed together (the single-problem drivers use
  * nprob = 1). */
@@ -83,6 +92,8 @@ int starsh_hip_gen_gemm_rw(int rw);
 int starsh_hip_qr_panel(int width);
 /* Projection of the mixed form on int8 digit planes (1) or FP64 MFMA (0). */
 int starsh_hip_proj_int8(int on);
+/* Sweep limit of the Jacobi core (1..60; 0 restores the default 60). */
+int starsh_hip_jacobi_sweeps(int sweeps);
 size_t hip_driver_workspace_budget(void);
 double hip_driver_now(void);
 

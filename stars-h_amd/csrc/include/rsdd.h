@@ -68,7 +68,8 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out);
  * to dst_v[t] for every tile t < ntiles of the last rsdd_batch with
  * dst_u[t] != NULL and rank > 0, in one device-to-host transfer. */
 int rsdd_copy_factors(const RsddOutput *out, int ntiles, const int *m,
-        const int *n, double *const *dst_u, double *const *dst_v);
+        const int *n, double *const *dst_u, double *const *dst_v,
+        double *d2h_seconds, double *scatter_seconds);
 
 /* Frees the workspace and staging buffers kept between calls. */
 void rsdd_release_pools(void);

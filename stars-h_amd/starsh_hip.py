@@ -120,7 +120,12 @@ class Stats(C.Structure):
                 ("ntiles", C.c_longlong), ("nfalse_far", C.c_longlong),
                 ("rank_min", C.c_int), ("rank_max", C.c_int),
                 ("rank_avg", C.c_double), ("flops_algorithmic", C.c_double),
-                ("sketch_bytes_or_flops", C.c_double)]
+                ("sketch_bytes_or_flops", C.c_double),
+                ("prepare_seconds", C.c_double),
+                ("batch_seconds", C.c_double),
+                ("d2h_seconds", C.c_double),
+                ("scatter_seconds", C.c_double),
+                ("finish_seconds", C.c_double)]
 
 
 class KernelDesc(C.Structure):

@@ -237,3 +237,32 @@ def test_gram_schmidt_panel_option():
         assert L.starsh_hip_set_option(b"qr_panel", C.c_double(w)) == 0
     assert L.starsh_hip_set_option(b"qr_panel", C.c_double(5)) != 0
     assert L.starsh_hip_set_option(b"qr_panel", C.c_double(0)) == 0
+
+
+@pytest.mark.parametrize("backend", ["HIP", "HIP_MIXED"])
+def test_oversized_tiles_rejected_before_gpu_work(backend):
+    """Tiles over 4096 rows/columns and sketch widths min(maxrank +
+    oversample, m, n) over 1024 are refused by argument checking, before the
+    driver touches the GPU (this container has none)."""
+    fn = S.init(backend, oversample=16)
+    # one 5000 x 5000 tile
+    p = S.TLRProblem("spatial", 5000, 2, 5000, symm="S", seed=0,
+                     kernel="exp", beta=0.1)
+    assert p.approximate(100, 1e-7, 0, fn) == S.STARSH_WRONG_PARAMETER
+    assert not p.M
+    # tiles of 2000 with maxrank 1100: sketch width 1116
+    p = S.TLRProblem("spatial", 4000, 2, 2000, symm="S", seed=0,
+                     kernel="exp", beta=0.1)
+    assert p.approximate(1100, 1e-7, 0, fn) == S.STARSH_WRONG_PARAMETER
+    assert not p.M
+    assert len(p.tiles()) == 3
+
+
+def test_jacobi_sweep_option_is_checked():
+    L = S.lib()
+    for bad in (-1, 61, 1000):
+        assert L.starsh_hip_set_option(b"jacobi_max_sweeps",
+                                       C.c_double(bad)) == \
+            S.STARSH_WRONG_PARAMETER
+    assert L.starsh_hip_set_option(b"jacobi_max_sweeps", C.c_double(5)) == 0
+    assert L.starsh_hip_set_option(b"jacobi_max_sweeps", C.c_double(0)) == 0
