@@ -128,7 +128,7 @@ def run_form(S, w, backend, steps, warmup, autotune):
         first, secs = ps[0].autotune(w["maxrank"], w["tol"])
         schedule = {"first_bucket_width": [32, 64, 128, 256, 512][first],
                     "candidate_device_seconds": [round(x, 6) for x in secs]}
-    dev, wall, dom = [], [], []
+    dev, wall, dom, proj = [], [], [], []
     parts = ("prepare", "batch", "d2h", "scatter", "finish")
     tl = {k: [] for k in parts}
     for it in range(warmup + steps):
@@ -143,6 +143,7 @@ def run_form(S, w, backend, steps, warmup, autotune):
             dev.append(st.device_seconds)
             wall.append(st.wall_seconds)
             dom.append(st.dominant_kernel_seconds)
+            proj.append(st.proj_kernel_seconds)
             for k in parts:
                 tl[k].append(getattr(st, k + "_seconds"))
         if it < warmup + steps - 1:
@@ -170,6 +171,9 @@ def run_form(S, w, backend, steps, warmup, autotune):
         # summed over the launches of one step
         "dominant_kernel_seconds": sum(dom) / len(dom),
         "dominant_kernel_launches": int(st.dominant_kernel_launches),
+        # summed over the launches of one step, HIP events
+        "sketch_kernel_seconds": sum(dom) / len(dom),
+        "proj_kernel_seconds": sum(proj) / len(proj),
         "sketch_flops": st.sketch_bytes_or_flops,
         "algorithmic_flops": st.flops_algorithmic,
         "tiles": int(st.ntiles),
@@ -187,27 +191,56 @@ def run_form(S, w, backend, steps, warmup, autotune):
     return res
 
 
-def cpu_baseline(w, p, nsample):
-    """The oracle (OpenMP restatement of the reference tile loop) on a
-    spread sample of the tiles, scaled to all tiles."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, p, nsample, repeats):
+    """The oracle (OpenMP restatement of the reference tile loop:
+    openmp/blrm/drsdd.c + sequential/dense/drsdd.c with the host LAPACK in
+    sequential mode) on every tile of the workload (nsample == 0) or on a
+    spread sample scaled to all tiles; one warm-up on a few tiles, then the
+    best of `repeats` timed runs, one OpenMP thread per host core."""
     from oracle import oracle as O
-    nthreads = min(16, os.cpu_count() or 1)
+    nthreads = os.cpu_count() or 1
     kd = O.kernel(w["kernel"], w["ndim"], beta=w.get("beta", 0.1),
                   nu=w.get("nu", 0.5), wave_k=w.get("wave_k", 1.0),
                   diag=w.get("diag", 1.0))
     tiles = p.tiles()
-    step = max(1, len(tiles) // nsample)
-    sample = tiles[::step][:nsample]
+    if nsample <= 0 or nsample >= len(tiles):
+        sample, step = tiles, 1
+    else:
+        step = max(1, len(tiles) // nsample)
+        sample = tiles[::step][:nsample]
     st, sz = p.cluster_ranges()
-    t0 = time.perf_counter()
-    O.compress(kd, p.points(), st, sz, sample, w["maxrank"], w["os"],
+    pts = p.points()
+    O.compress(kd, pts, st, sz, sample[:nthreads], w["maxrank"], w["os"],
                w["tol"], nthreads=nthreads, want_factors=False)
-    dt = time.perf_counter() - t0
-    return {"value": dt * len(tiles) / len(sample), "unit": "s",
+    times = []
+    for _ in range(max(1, repeats)):
+        t0 = time.perf_counter()
+        O.compress(kd, pts, st, sz, sample, w["maxrank"], w["os"],
+                   w["tol"], nthreads=nthreads, want_factors=False)
+        times.append(time.perf_counter() - t0)
+    dt = min(times)
+    scaled = len(sample) < len(tiles)
+    what = (f"all {len(tiles)} tiles" if not scaled else
+            f"{len(sample)} of {len(tiles)} tiles (every {step}th), scaled "
+            f"to all tiles")
+    return {"value": round(dt * len(tiles) / len(sample), 4), "unit": "s",
             "cores": nthreads, "kind": "port",
-            "sample": f"{len(sample)} of {len(tiles)} tiles (every {step}th)"
-                      f" compressed in {dt:.2f} s with {nthreads} OpenMP "
-                      f"threads, MKL sequential; scaled to all tiles",
+            "sample": f"{what}; best of {len(times)} runs "
+                      f"({', '.join(f'{x:.2f}' for x in times)} s) after a "
+                      f"warm-up, {nthreads} OpenMP threads (one per host "
+                      f"core), LAPACK sequential",
+            "cpu_model": _cpu_model(),
             "lapack": O.lib().oracle_lapack_name().decode()}
 
 
@@ -217,7 +250,10 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="C2", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-sample", type=int, default=48)
+    ap.add_argument("--cpu-sample", type=int, default=None,
+                    help="tiles of the CPU baseline (0: all; default all "
+                         "for C2, 256 otherwise)")
+    ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="plain,mixed")
     ap.add_argument("--no-autotune", action="store_true")
@@ -267,7 +303,10 @@ def main():
     cpu = None
     if not args.no_cpu and plain is not None and w["kind"] != "batch":
         print("bench: CPU baseline sample", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(w, plain["problems"][0], args.cpu_sample)
+        ns = args.cpu_sample
+        if ns is None:
+            ns = 0 if args.config == "C2" else 256
+        cpu = cpu_baseline(w, plain["problems"][0], ns, args.cpu_repeats)
     # attainable FP64 rates of this GPU (MFMA chains, vector FMA)
     mf, vf = ctypes.c_double(0.0), ctypes.c_double(0.0)
     probe = None

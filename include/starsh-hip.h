@@ -94,6 +94,8 @@ typedef struct starsh_hip_stats
     double d2h_seconds;      /* pack + device-to-host copy of U and V */
     double scatter_seconds;  /* host copy into the caller's buffers */
     double finish_seconds;   /* false far-field rewrite, near tiles, blrm */
+    double proj_kernel_seconds; /* summed duration of the projection
+                                   kernels B^T = A^T Q (HIP events) */
 } starsh_hip_stats;
 
 int starsh_hip_get_stats(starsh_hip_stats *stats);

@@ -13,6 +13,7 @@
  *      (:305-306).
  * The mixed-precision driver shares steps 1 and 3-5 (hip_driver_finish). */
 #include <time.h>
+#include <sys/mman.h>
 #include "common.h"
 #include "kernels_host.h"
 #include "rsdd.h"
@@ -199,13 +200,17 @@ void hip_driver_release(hip_problem *hp)
     hp->d_xr = hp->d_xc = NULL;
 }
 
-/* Steps 1: host buffers and wrappers (drsdd.c:53-90).  hf owns what it
- * points to; on failure hip_driver_free_factors releases it. */
+/* Step 1 (drsdd.c:53-90), deferred: the reference allocates U and V
+ * maxrank columns wide per tile before compressing.  Here the factors stay
+ * on the device until every tile is done, and hip_driver_download then
+ * allocates host buffers of the exact sizes (U_t m x r, V_t n x r, packed).
+ * hf owns what it points to; on failure hip_driver_free_factors releases
+ * it. */
 int hip_driver_alloc_factors(STARSH_blrf *F, int maxrank, hip_factors *hf)
 {
+    (void)maxrank;
     memset(hf, 0, sizeof(*hf));
     STARSH_int nfar = F->nblocks_far;
-    STARSH_cluster *RC = F->row_cluster, *CC = F->col_cluster;
     if(nfar == 0)
         return STARSH_SUCCESS;
     hf->far_U = calloc(nfar, sizeof(Array *));
@@ -216,29 +221,125 @@ int hip_driver_alloc_factors(STARSH_blrf *F, int maxrank, hip_factors *hf)
     STARSH_MALLOC(hf->voff, nfar);
     if(hf->far_U == NULL || hf->far_V == NULL)
         return STARSH_MALLOC_ERROR;
-    size_t su = 0, sv = 0;
     for(STARSH_int bi = 0; bi < nfar; bi++)
     {
-        hf->uoff[bi] = su;
-        hf->voff[bi] = sv;
-        su += (size_t)RC->size[F->block_far[2*bi]]*maxrank;
-        sv += (size_t)CC->size[F->block_far[2*bi+1]]*maxrank;
-    }
-    STARSH_MALLOC(hf->alloc_U, su);
-    STARSH_MALLOC(hf->alloc_V, sv);
-    for(STARSH_int bi = 0; bi < nfar; bi++)
-    {
-        int shape_U[2] = {(int)RC->size[F->block_far[2*bi]], maxrank};
-        int shape_V[2] = {(int)CC->size[F->block_far[2*bi+1]], maxrank};
-        int info = array_from_buffer(hf->far_U+bi, 2, shape_U, 'd', 'F',
-                hf->alloc_U+hf->uoff[bi]);
-        if(info == STARSH_SUCCESS)
-            info = array_from_buffer(hf->far_V+bi, 2, shape_V, 'd', 'F',
-                    hf->alloc_V+hf->voff[bi]);
-        if(info != STARSH_SUCCESS)
-            return info;
+        hf->far_rank[bi] = 0;
+        hf->uoff[bi] = hf->voff[bi] = -1;
     }
     return STARSH_SUCCESS;
+}
+
+/* Host buffer for factors: 2 MiB aligned and, when large, backed by huge
+ * pages where the kernel allows it, so that first touches fault 2 MiB at a
+ * time.  Released with free(), as the reference's malloc'd buffers. */
+static double *alloc_host(size_t n)
+{
+    void *p = NULL;
+    const size_t bytes = sizeof(double)*(n > 0 ? n : 1);
+    if(posix_memalign(&p, (size_t)2 << 20, bytes) != 0)
+        return NULL;
+    if(bytes >= ((size_t)4 << 20))
+        (void)madvise(p, bytes, MADV_HUGEPAGE);
+    return p;
+}
+
+typedef struct
+{
+    int64_t aoff, len;
+    double *dst;
+} seg_t;
+
+static int seg_cmp(const void *a, const void *b)
+{
+    const int64_t x = ((const seg_t *)a)->aoff, y = ((const seg_t *)b)->aoff;
+    return x < y ? -1 : x > y;
+}
+
+/* Step 1 done late, and the copy-back: host buffers of the exact sizes for
+ * every problem of the batch, one download of the device arena into them,
+ * and the U/V wrappers (shape m x r, n x r). */
+int hip_driver_download(const hip_batch *B, starsh_hip_stats *st)
+{
+    int64_t nseg = 0;
+    for(int q = 0; q < B->nprob; q++)
+        nseg += 2*B->hf[q].nfar;
+    seg_t *seg = malloc(sizeof(seg_t)*(nseg+1));
+    if(seg == NULL)
+        return STARSH_MALLOC_ERROR;
+    nseg = 0;
+    for(int q = 0; q < B->nprob; q++)
+    {
+        hip_factors *hf = B->hf+q;
+        const STARSH_blrf *F = B->F[q];
+        const STARSH_cluster *RC = F->row_cluster, *CC = F->col_cluster;
+        if(hf->nfar == 0)
+            continue;
+        size_t su = 0, sv = 0;
+        for(STARSH_int bi = 0; bi < hf->nfar; bi++)
+        {
+            const int r = hf->far_rank[bi];
+            if(r <= 0)
+                continue;
+            su += (size_t)RC->size[F->block_far[2*bi]]*r;
+            sv += (size_t)CC->size[F->block_far[2*bi+1]]*r;
+        }
+        hf->alloc_U = alloc_host(su);
+        hf->alloc_V = alloc_host(sv);
+        if(hf->alloc_U == NULL || hf->alloc_V == NULL)
+        {
+            free(seg);
+            STARSH_ERROR("malloc() failed");
+            return STARSH_MALLOC_ERROR;
+        }
+        size_t ou = 0, ov = 0;
+        for(STARSH_int bi = 0; bi < hf->nfar; bi++)
+        {
+            const int r = hf->far_rank[bi];
+            if(r < 0)
+                continue;
+            const int m = (int)RC->size[F->block_far[2*bi]];
+            const int n = (int)CC->size[F->block_far[2*bi+1]];
+            int shape_U[2] = {m, r}, shape_V[2] = {n, r};
+            int info = array_from_buffer(hf->far_U+bi, 2, shape_U, 'd', 'F',
+                    hf->alloc_U+ou);
+            if(info == STARSH_SUCCESS)
+                info = array_from_buffer(hf->far_V+bi, 2, shape_V, 'd', 'F',
+                        hf->alloc_V+ov);
+            if(info != STARSH_SUCCESS)
+            {
+                free(seg);
+                return info;
+            }
+            if(r > 0)
+            {
+                seg[nseg++] = (seg_t){hf->uoff[bi], (int64_t)m*r,
+                    hf->alloc_U+ou};
+                seg[nseg++] = (seg_t){hf->voff[bi], (int64_t)n*r,
+                    hf->alloc_V+ov};
+            }
+            ou += (size_t)m*r;
+            ov += (size_t)n*r;
+        }
+    }
+    qsort(seg, nseg, sizeof(seg_t), seg_cmp);
+    int64_t *aoff = malloc(sizeof(int64_t)*(nseg+1));
+    int64_t *len = malloc(sizeof(int64_t)*(nseg+1));
+    double **dst = malloc(sizeof(double *)*(nseg+1));
+    int info = aoff && len && dst ? STARSH_SUCCESS : STARSH_MALLOC_ERROR;
+    for(int64_t q = 0; info == STARSH_SUCCESS && q < nseg; q++)
+    {
+        aoff[q] = seg[q].aoff;
+        len[q] = seg[q].len;
+        dst[q] = seg[q].dst;
+    }
+    if(info == STARSH_SUCCESS)
+        info = rsdd_arena_download(nseg, aoff, len, dst, &st->d2h_seconds,
+                &st->scatter_seconds);
+    free(seg);
+    free(aoff);
+    free(len);
+    free(dst);
+    return info;
 }
 
 /* Releases what hf holds.  The wrappers only view alloc_U/alloc_V, so
@@ -377,35 +478,20 @@ int hip_driver_finish(STARSH_blrm **matrix, STARSH_blrf *format,
         STARSH_WARNING("`F` was modified due to false far-field blocks");
         starsh_blrf_free(F2);
         block_near = F->block_near;
-        /* Compaction of the far-field arrays (drsdd.c:259-282); the
-         * wrappers of false far-field tiles are released. */
+        /* Compaction of the far-field arrays (drsdd.c:259-282): false
+         * far-field tiles have no wrappers (rank -1, no factors). */
         for(bi = 0, bj = 0; bi < nblocks_far; bi++)
         {
-            Array *u = hf->far_U[bi], *v = hf->far_V[bi];
-            hf->far_U[bi] = hf->far_V[bi] = NULL;
-            void *du = u->data, *dv = v->data;
-            int shape_U[2] = {u->shape[0], hf->far_rank[bi]};
-            int shape_V[2] = {v->shape[0], hf->far_rank[bi]};
-            u->data = NULL;
-            array_free(u);
-            v->data = NULL;
-            array_free(v);
             if(hf->far_rank[bi] == -1)
             {
                 bj++;
                 continue;
             }
-            info = array_from_buffer(hf->far_U+bi-bj, 2, shape_U, 'd', 'F',
-                    du);
-            if(info == STARSH_SUCCESS)
-                info = array_from_buffer(hf->far_V+bi-bj, 2, shape_V, 'd',
-                        'F', dv);
+            hf->far_U[bi-bj] = hf->far_U[bi];
+            hf->far_V[bi-bj] = hf->far_V[bi];
             hf->far_rank[bi-bj] = hf->far_rank[bi];
-            if(info != STARSH_SUCCESS)
-            {
-                hf->nfar = bi-bj+1;
-                return info;
-            }
+            if(bj > 0)
+                hf->far_U[bi] = hf->far_V[bi] = NULL;
         }
         hf->nfar = new_far;
         if(new_far == 0)
@@ -538,11 +624,11 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
     int64_t *uo = malloc(sizeof(int64_t)*ntiles);
     int64_t *vo = malloc(sizeof(int64_t)*ntiles);
     RsddProblem *probs = malloc(sizeof(RsddProblem)*B->nprob);
-    double **du = malloc(sizeof(double *)*ntiles);
-    double **dv = malloc(sizeof(double *)*ntiles);
+    int64_t *au = malloc(sizeof(int64_t)*ntiles);
+    int64_t *av = malloc(sizeof(int64_t)*ntiles);
     int64_t *so = malloc(sizeof(int64_t)*ntiles);
-    if(!m || !n || !pp || !kc || !r0 || !c0 || !uo || !vo || !probs || !du ||
-            !dv || !so)
+    if(!m || !n || !pp || !kc || !r0 || !c0 || !uo || !vo || !probs || !au ||
+            !av || !so)
         info = STARSH_MALLOC_ERROR;
     for(STARSH_int t = 0; info == STARSH_SUCCESS && t < ntiles; t++)
         so[t] = (int64_t)t*sstride;
@@ -625,6 +711,7 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
             break;
         st->device_seconds += out.device_seconds;
         st->dominant_kernel_seconds += out.sketch_seconds;
+        st->proj_kernel_seconds += out.proj_seconds;
         st->dominant_kernel_launches += 1;
         double tc = hip_driver_now();
         for(STARSH_int t = b0; t < b1; t++)
@@ -636,24 +723,26 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
             kfull = kfull < mn ? kfull : mn;
             accepted[t] = accept ? accept(rank_out[t], kk, kfull,
                     flags_out[t], B->hp[q].oversample) : 1;
-            du[t] = dv[t] = NULL;
+        }
+        /* factors of the finished tiles to the device arena; the host
+         * copy happens once all tiles are done (hip_driver_download) */
+        info = rsdd_arena_append(&out, (int)(b1-b0), m+b0, n+b0, accepted+b0,
+                au+b0, av+b0);
+        for(STARSH_int t = b0; info == STARSH_SUCCESS && t < b1; t++)
+        {
             if(!accepted[t])
                 continue;
-            hip_factors *hf = B->hf+q;
-            STARSH_int bi = idx[t];
+            hip_factors *hf = B->hf+pidx[t];
+            const STARSH_int bi = idx[t];
             hf->far_rank[bi] = rank_out[t];
-            if(rank_out[t] <= 0)
-                continue;
-            du[t] = hf->alloc_U+hf->uoff[bi];
-            dv[t] = hf->alloc_V+hf->voff[bi];
+            hf->uoff[bi] = au[t];
+            hf->voff[bi] = av[t];
         }
-        info = rsdd_copy_factors(&out, (int)(b1-b0), m+b0, n+b0, du+b0,
-                dv+b0, &st->d2h_seconds, &st->scatter_seconds);
         st->copy_seconds += hip_driver_now()-tc;
         b0 = b1;
     }
     free(m); free(n); free(pp); free(kc); free(r0); free(c0); free(uo);
-    free(vo); free(probs); free(du); free(dv); free(so);
+    free(vo); free(probs); free(au); free(av); free(so);
     return info;
 }
 
@@ -732,6 +821,8 @@ int hip_compress(int mixed, int nprob, STARSH_blrm **matrix,
     memset(&st, 0, sizeof(st));
     int info = B.hp && B.hf ? STARSH_SUCCESS : STARSH_MALLOC_ERROR;
     int nprep = 0;
+    if(info == STARSH_SUCCESS)
+        info = rsdd_arena_reset();
     for(int q = 0; info == STARSH_SUCCESS && q < nprob; q++)
     {
         info = hip_driver_prepare(format[q], maxrank[q], tol[q], onfly,
@@ -771,6 +862,12 @@ int hip_compress(int mixed, int nprob, STARSH_blrm **matrix,
             hip_plain_tiles(&B, nt, pidx, idx, &st);
     }
     free(pidx); free(idx); free(done);
+    if(info == STARSH_SUCCESS)
+    {
+        double td = hip_driver_now();
+        info = hip_driver_download(&B, &st);
+        st.copy_seconds += hip_driver_now()-td;
+    }
     if(info == STARSH_SUCCESS)
         hip_driver_rank_stats(&B, &st);
     double tf = hip_driver_now();

@@ -402,7 +402,8 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     // destroyed on every return path
     struct Events
     {
-        hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+        hipEvent_t e[6] = {nullptr, nullptr, nullptr, nullptr, nullptr,
+            nullptr};
         ~Events()
         {
             for(auto x : e)
@@ -411,7 +412,7 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
         }
     } evs;
     hipEvent_t *ev = evs.e;
-    for(int e = 0; e < 4; e++)
+    for(int e = 0; e < 6; e++)
         HIP_TRY(hipEventCreate(&ev[e]));
     HIP_TRY(hipEventRecord(ev[0], s));
     // 1. sketch
@@ -442,17 +443,21 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     TRY(pipeline_compact(d_tiles, nt, d_ws, (int *)keep.p, kmax, s));
     dump("Q", d_ws+h[0].q_off, (int64_t)h[0].m*h[0].k, s);
     // 3. projection Bt = A^T Q
+    HIP_TRY(hipEventRecord(ev[4], s));
     if(in->d_a)
         TRY(pipeline_gemm(GM_PROJ_MEM, d_tiles, nt, nmax, kmax, g, s));
     else if(proj8)
     {
         TRY(pipeline_qdigits(d_tiles, nt, kmax, d_ws, (signed char *)qd.p,
                     s));
+        // the projection kernel alone
+        HIP_TRY(hipEventRecord(ev[4], s));
         TRY(pipeline_proj_i8(d_tiles, nt, nmax, kmax, g,
                     (const signed char *)qd.p, s));
     }
     else
         TRY(pipeline_gen_gemm(1, d_tiles, nt, nmax, kmax, g, s));
+    HIP_TRY(hipEventRecord(ev[5], s));
     dump("Bt", d_ws+h[0].bt_off, (int64_t)h[0].n*h[0].k, s);
     // 4. truncation of B's rows
     TRY(pipeline_colnorm(d_tiles, nt, 2, d_ws, (double *)norms.p, kmax, s));
@@ -497,6 +502,9 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     float ms = 0, ms_sk = 0;
     HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
     HIP_TRY(hipEventElapsedTime(&ms_sk, ev[2], ev[3]));
+    float ms_pr = 0;
+    HIP_TRY(hipEventElapsedTime(&ms_pr, ev[4], ev[5]));
+    out->proj_seconds = ms_pr*1e-3;
     out->device_seconds = ms*1e-3;
     out->sketch_seconds = ms_sk*1e-3;
     for(int t = 0; t < nt; t++)
@@ -511,66 +519,158 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
 }
 
 
-// Copies the factors of the tiles with dst_u[t] != NULL to host memory:
-// U_t (m[t] x rank[t]) to dst_u[t], V_t (n[t] x rank[t]) to dst_v[t].  The
-// segments are packed on the device, brought over in one copy through
-// pinned memory, and scattered on the host by several threads.
-int rsdd_copy_factors(const RsddOutput *out, int ntiles, const int *m,
-        const int *n, double *const *dst_u, double *const *dst_v,
-        double *d2h_seconds, double *scatter_seconds)
+// Arena of the factors of one API call (device, grow-only, kept between
+// calls) and its fill level in doubles.
+static Pool arena_pool;
+static int64_t arena_used = 0;
+static Pool seg_pool;
+
+int rsdd_arena_reset(void)
 {
-    const auto t0 = std::chrono::steady_clock::now();
+    arena_used = 0;
+    return STARSH_SUCCESS;
+}
+
+int rsdd_arena_append(const RsddOutput *out, int ntiles, const int *m,
+        const int *n, const int *take, int64_t *uo, int64_t *vo)
+{
     hipStream_t s = starsh_hip_stream();
     std::vector<int64_t> seg;
-    std::vector<double *> dst;
-    int64_t total = 0;
+    int64_t add = 0;
     for(int t = 0; t < ntiles; t++)
     {
-        if(dst_u[t] == nullptr || out->rank[t] <= 0)
+        uo[t] = vo[t] = -1;
+        if(!take[t] || out->rank[t] <= 0)
             continue;
         const int64_t lu = (int64_t)m[t]*out->rank[t];
         const int64_t lv = (int64_t)n[t]*out->rank[t];
-        seg.insert(seg.end(), {out->uoff[t], total, lu});
-        dst.push_back(dst_u[t]);
-        total += lu;
-        seg.insert(seg.end(), {out->voff[t], total, lv});
-        dst.push_back(dst_v[t]);
-        total += lv;
+        uo[t] = arena_used+add;
+        seg.insert(seg.end(), {out->uoff[t], uo[t], lu});
+        add += lu;
+        vo[t] = arena_used+add;
+        seg.insert(seg.end(), {out->voff[t], vo[t], lv});
+        add += lv;
     }
-    const int nseg = (int)dst.size();
+    const int nseg = (int)(seg.size()/3);
     if(nseg == 0)
         return STARSH_SUCCESS;
-    TRY(pack_pool.reserve(sizeof(double)*total+sizeof(int64_t)*seg.size()));
-    TRY(stage_pool.reserve(sizeof(double)*total));
-    double *d_pack = (double *)pack_pool.p;
-    int64_t *d_seg = (int64_t *)(d_pack+total);
+    const size_t need = sizeof(double)*(size_t)(arena_used+add);
+    if(need > arena_pool.bytes)
+    {
+        // grow: a new buffer with the old contents copied over
+        Pool grown;
+        TRY(grown.reserve(std::max(need, arena_pool.bytes+arena_pool.bytes/2)));
+        if(arena_used > 0)
+            HIP_TRY(hipMemcpyAsync(grown.p, arena_pool.p,
+                        sizeof(double)*(size_t)arena_used,
+                        hipMemcpyDeviceToDevice, s));
+        arena_pool.release();
+        arena_pool.p = grown.p;
+        arena_pool.bytes = grown.bytes;
+        grown.p = nullptr;
+        grown.bytes = 0;
+    }
+    TRY(seg_pool.reserve(sizeof(int64_t)*seg.size()));
+    int64_t *d_seg = (int64_t *)seg_pool.p;
     HIP_TRY(hipMemcpyAsync(d_seg, seg.data(), sizeof(int64_t)*seg.size(),
                 hipMemcpyHostToDevice, s));
     for(int s0 = 0; s0 < nseg; s0 += 65535)
     {
         const int cnt = std::min(nseg-s0, 65535);
         hipLaunchKernelGGL(pack_kernel, dim3(16, cnt), dim3(256), 0, s,
-                out->d_uv, d_pack, d_seg+3*(int64_t)s0, cnt);
+                out->d_uv, (double *)arena_pool.p, d_seg+3*(int64_t)s0,
+                cnt);
         HIP_TRY(hipGetLastError());
     }
-    double *h = (double *)stage_pool.p;
-    HIP_TRY(hipMemcpyAsync(h, d_pack, sizeof(double)*total,
-                hipMemcpyDeviceToHost, s));
+    // the segment list is host memory of this call: wait until the
+    // transfer has read it
     HIP_TRY(hipStreamSynchronize(s));
-    const auto t1 = std::chrono::steady_clock::now();
-    // host scatter: segments split over threads by size
+    arena_used += add;
+    return STARSH_SUCCESS;
+}
+
+// Events of the chunked download, created once.
+static std::vector<hipEvent_t> chunk_events;
+
+int rsdd_arena_download(int64_t nseg, const int64_t *aoff,
+        const int64_t *len, double *const *dst, double *d2h_seconds,
+        double *scatter_seconds)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = starsh_hip_stream();
+    if(nseg <= 0)
+        return STARSH_SUCCESS;
+    const int64_t lo = aoff[0], hi = aoff[nseg-1]+len[nseg-1];
+    const int64_t total = hi-lo;
+    if(total <= 0)
+        return STARSH_SUCCESS;
+    TRY(stage_pool.reserve(sizeof(double)*(size_t)total));
+    double *h = (double *)stage_pool.p;
+    const double *d = (const double *)arena_pool.p+lo;
+    // chunks of 4 Mi doubles (32 MiB)
+    const int64_t CH = (int64_t)1 << 22;
+    const int nch = (int)((total+CH-1)/CH);
+    while((int)chunk_events.size() < nch)
+    {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        chunk_events.push_back(e);
+    }
+    for(int c = 0; c < nch; c++)
+    {
+        const int64_t c0 = (int64_t)c*CH, c1 = std::min(total, c0+CH);
+        HIP_TRY(hipMemcpyAsync(h+c0, d+c0, sizeof(double)*(size_t)(c1-c0),
+                    hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipEventRecord(chunk_events[c], s));
+    }
+    // Host threads: touch their share of the destination pages (the first
+    // touch maps and zeroes them) while the transfer runs, then copy each
+    // chunk as soon as it has arrived, every thread taking 1/nth of it.
     unsigned nth = std::thread::hardware_concurrency();
     nth = nth < 1 ? 1 : (nth > 16 ? 16 : nth);
-    if(total < ((int64_t)1 << 20))
+    if(total < ((int64_t)1 << 18))
         nth = 1;
-    auto work = [&](unsigned id) {
-        const int64_t lo = total*id/nth, hi = total*(id+1)/nth;
-        for(int q = 0; q < nseg; q++)
+    std::vector<int> fail(nth, 0);
+    auto copy_range = [&](int64_t a, int64_t b) {
+        // segments are in increasing arena order: find the first one
+        // ending after a
+        int64_t q = std::upper_bound(aoff, aoff+nseg, lo+a)-aoff-1;
+        q = q < 0 ? 0 : q;
+        for(; q < nseg && aoff[q]-lo < b; q++)
         {
-            const int64_t po = seg[3*q+1], len = seg[3*q+2];
-            const int64_t a = std::max(lo, po), b = std::min(hi, po+len);
-            if(a < b)
-                memcpy(dst[q]+(a-po), h+a, sizeof(double)*(b-a));
+            const int64_t so = aoff[q]-lo;
+            const int64_t x = std::max(a, so), y = std::min(b, so+len[q]);
+            if(x < y)
+                memcpy(dst[q]+(x-so), h+x, sizeof(double)*(size_t)(y-x));
+        }
+    };
+    auto touch_range = [&](int64_t a, int64_t b) {
+        int64_t q = std::upper_bound(aoff, aoff+nseg, lo+a)-aoff-1;
+        q = q < 0 ? 0 : q;
+        for(; q < nseg && aoff[q]-lo < b; q++)
+        {
+            const int64_t so = aoff[q]-lo;
+            const int64_t x = std::max(a, so), y = std::min(b, so+len[q]);
+            volatile char *p = (volatile char *)(dst[q]+(x-so));
+            const int64_t nbytes = sizeof(double)*(y-x);
+            for(int64_t o = 0; o < nbytes; o += 4096)
+                p[o] = 0;
+        }
+    };
+    auto t_last = t0;
+    auto work = [&](unsigned id) {
+        touch_range(total*id/nth, total*(id+1)/nth);
+        for(int c = 0; c < nch; c++)
+        {
+            if(hipEventSynchronize(chunk_events[c]) != hipSuccess)
+            {
+                fail[id] = 1;
+                return;
+            }
+            if(id == 0 && c == nch-1)
+                t_last = std::chrono::steady_clock::now();
+            const int64_t c0 = (int64_t)c*CH, c1 = std::min(total, c0+CH);
+            copy_range(c0+(c1-c0)*id/nth, c0+(c1-c0)*(id+1)/nth);
         }
     };
     std::vector<std::thread> th;
@@ -579,9 +679,20 @@ int rsdd_copy_factors(const RsddOutput *out, int ntiles, const int *m,
     work(0);
     for(auto &x : th)
         x.join();
-    const auto t2 = std::chrono::steady_clock::now();
-    *d2h_seconds += std::chrono::duration<double>(t1-t0).count();
-    *scatter_seconds += std::chrono::duration<double>(t2-t1).count();
+    const auto t1 = std::chrono::steady_clock::now();
+    for(int f : fail)
+        if(f)
+        {
+            fprintf(stderr, "STARSH HIP ERROR: device-to-host copy of the "
+                    "factors failed\n");
+            return STARSH_UNKNOWN_ERROR;
+        }
+    // d2h: until the last chunk has arrived (the host copies of the
+    // earlier chunks overlap it); scatter: the host work left after that
+    if(t_last < t0)
+        t_last = t0;
+    *d2h_seconds += std::chrono::duration<double>(t_last-t0).count();
+    *scatter_seconds += std::chrono::duration<double>(t1-t_last).count();
     return STARSH_SUCCESS;
 }
 
@@ -592,6 +703,9 @@ void rsdd_release_pools(void)
     uv_pool.release();
     pack_pool.release();
     stage_pool.release();
+    arena_pool.release();
+    seg_pool.release();
+    arena_used = 0;
 }
 
 extern "C" int starsh_hip_proj_int8(int on)

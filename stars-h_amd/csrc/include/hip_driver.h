@@ -22,7 +22,8 @@ typedef struct hip_factors
     Array **far_U, **far_V;
     int *far_rank;
     double *alloc_U, *alloc_V;
-    size_t *uoff, *voff;   /* offsets of each tile in alloc_U / alloc_V */
+    int64_t *uoff, *voff;  /* arena offsets of each tile's U and V (-1:
+                              none) until the download */
     STARSH_int nfar;       /* entries of far_U, far_V and far_rank */
 } hip_factors;
 
@@ -68,6 +69,9 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
         int *rank_out, int *flags_out, int *accepted, hip_accept_fn accept,
         double *S_out, int sstride, starsh_hip_stats *st);
 void hip_driver_rank_stats(const hip_batch *B, starsh_hip_stats *st);
+/* After the last hip_driver_run of a batch: host U/V buffers of the exact
+ * sizes for every problem, filled from the device arena, with wrappers. */
+int hip_driver_download(const hip_batch *B, starsh_hip_stats *st);
 
 /* The plain and mixed forms over a list of tiles of a batch. */
 int hip_plain_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,

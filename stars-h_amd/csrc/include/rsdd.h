@@ -59,17 +59,36 @@ typedef struct rsdd_output
                                     the pipeline and valid until its next
                                     call */
     double device_seconds;
-    double sketch_seconds;
+    double sketch_seconds;       /* sketch kernel (HIP events) */
+    double proj_seconds;         /* projection kernels (HIP events) */
 } RsddOutput;
 
 int rsdd_batch(const RsddInput *in, RsddOutput *out);
 
-/* Copies U_t (m[t] x rank[t], ld m[t]) to dst_u[t] and V_t (n[t] x rank[t])
- * to dst_v[t] for every tile t < ntiles of the last rsdd_batch with
- * dst_u[t] != NULL and rank > 0, in one device-to-host transfer. */
-int rsdd_copy_factors(const RsddOutput *out, int ntiles, const int *m,
-        const int *n, double *const *dst_u, double *const *dst_v,
-        double *d2h_seconds, double *scatter_seconds);
+/* Device arena of the factors of one API call.  Tiles are compressed in
+ * several batches (workspace chunks, rank rounds); the factors of the
+ * tiles each batch finishes are appended to the arena, and one download at
+ * the end of the call brings them all to the caller's host buffers, when
+ * every rank (so every buffer size) is known.  The arena is kept between
+ * calls (grow-only).
+ *   rsdd_arena_reset   empties it (start of a call);
+ *   rsdd_arena_append  appends U_t then V_t of every tile t < ntiles of
+ *                      the last rsdd_batch with take[t] != 0 and rank > 0;
+ *                      their arena offsets (doubles) go to uo[t], vo[t]
+ *                      (-1 for the other tiles);
+ *   rsdd_arena_download copies nseg arena segments (offset aoff[q], len[q]
+ *                      doubles) to host dst[q]: device-to-host in chunks
+ *                      through pinned memory, the copy into dst overlapped
+ *                      with the transfer of the next chunk by host threads,
+ *                      which first touch the destination pages while the
+ *                      transfer starts.  The segments must be in increasing
+ *                      arena order. */
+int rsdd_arena_reset(void);
+int rsdd_arena_append(const RsddOutput *out, int ntiles, const int *m,
+        const int *n, const int *take, int64_t *uo, int64_t *vo);
+int rsdd_arena_download(int64_t nseg, const int64_t *aoff,
+        const int64_t *len, double *const *dst, double *d2h_seconds,
+        double *scatter_seconds);
 
 /* Frees the workspace and staging buffers kept between calls. */
 void rsdd_release_pools(void);

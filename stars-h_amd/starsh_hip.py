@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(HERE, "libstarsh_hip.so")
 
 # enum values (include/starsh-constants.h and application headers)
 STARSH_SUCCESS = 0
+STARSH_MALLOC_ERROR = 1
 STARSH_WRONG_PARAMETER = 4
 STARSH_SPATIAL = 3
 STARSH_ELECTROSTATICS = 5
@@ -125,7 +126,8 @@ class Stats(C.Structure):
                 ("batch_seconds", C.c_double),
                 ("d2h_seconds", C.c_double),
                 ("scatter_seconds", C.c_double),
-                ("finish_seconds", C.c_double)]
+                ("finish_seconds", C.c_double),
+                ("proj_kernel_seconds", C.c_double)]
 
 
 class KernelDesc(C.Structure):
