@@ -132,6 +132,20 @@ int starsh_hip_tlr_bench(starsh_hip_tlr *T, int nrhs, int reps,
         double *seconds, double *bytes);
 void starsh_hip_tlr_free(starsh_hip_tlr *T);
 
+/* Allocation-free variant of starsh_hip_tlr_dmml.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * It computes the same product with the same kernels and allocates nothing:
+ * the r x nrhs coefficient blocks of the far-field tiles, which
+ * starsh_hip_tlr_dmml keeps in a workspace of T grown on demand, go to a
+ * static device array of 2^22 doubles (32 MiB) that each GPU holds until
+ * the process ends.  Products whose coefficients need more than that,
+ * (2 x sum of far-field ranks) x nrhs doubles, return
+ * STARSH_WRONG_PARAMETER.  All calls on one GPU share the array: calls that
+ * overlap on one GPU (from several host threads) race. */
+int starsh_hip_tlr_dmml_noalloc(starsh_hip_tlr *T, int nrhs, double alpha,
+        const double *d_A, int lda, double beta, double *d_B, int ldb);
+
 /* Measured FP64 rates of the current GPU in TFLOP/s (no reference
  * counterpart): independent v_mfma_f64_16x16x4f64 chains on every SIMD,
  * and FP64 fused multiply-adds on the vector ALUs.  For reading the

@@ -2,6 +2,7 @@
 // matrix and dense tile generation.
 #include <stdio.h>
 #include "hip_common.h"
+#include "tlr.h"
 
 static hipStream_t g_stream = nullptr;
 static int g_device = -1;
@@ -26,8 +27,18 @@ extern "C" int starsh_hip_init(int device)
     HIP_TRY(hipSetDevice(device));
     if(g_stream == nullptr)
         HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    // device address of the allocation-free product's scratch, once per
+    // device, with the device current
+    const int info = tlr_scratch_setup(device);
+    if(info != STARSH_SUCCESS)
+        return info;
     g_device = device;
     return STARSH_SUCCESS;
+}
+
+extern "C" int starsh_hip_current_device(void)
+{
+    return g_device;
 }
 
 extern "C" int starsh_hip_malloc(void **d_ptr, size_t bytes)

@@ -13,6 +13,7 @@
 //     contributions to those rows (U T, V T', D X_j, D^T X_i) in a fixed
 //     order: lane = row, wave w takes entries w, w+4, ... of the block row,
 //     and the four partial sums are added in LDS in wave order.
+#include <stdio.h>
 #include "hip_common.h"
 #include "tlr.h"
 
@@ -233,6 +234,43 @@ inline unsigned vec_grid(int64_t n)
 
 } // namespace
 
+
+// Scratch of the allocation-free product (starsh_hip_tlr_dmml_noalloc): the
+// coefficient blocks V^T X_j and U^T X_i of one call, in a static device
+// array that each GPU holds from the load of the library to the end of the
+// process.  Host code never uses the array's name as an address: its
+// device address is looked up once per device (tlr_scratch_setup, called
+// by starsh_hip_init with that device current) and kept in scratch_ptr.
+static __device__ __align__(256) double scratch[TLR_SCRATCH_DOUBLES];
+static double *scratch_ptr[TLR_MAX_DEVICES];
+
+extern "C" int tlr_scratch_setup(int device)
+{
+    if(device < 0 || device >= TLR_MAX_DEVICES)
+        return STARSH_WRONG_PARAMETER;
+    if(scratch_ptr[device] != nullptr)
+        return STARSH_SUCCESS;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if(e != hipSuccess || p == nullptr)
+    {
+        fprintf(stderr, "STARSH HIP ERROR: hipGetSymbolAddress of the "
+                "product scratch failed on device %d: %s\n", device,
+                hipGetErrorString(e));
+        return STARSH_UNKNOWN_ERROR;
+    }
+    scratch_ptr[device] = (double *)p;
+    return STARSH_SUCCESS;
+}
+
+extern "C" int tlr_scratch(int device, double **ptr)
+{
+    if(device < 0 || device >= TLR_MAX_DEVICES ||
+            scratch_ptr[device] == nullptr)
+        return STARSH_UNKNOWN_ERROR;
+    *ptr = scratch_ptr[device];
+    return STARSH_SUCCESS;
+}
 
 int starsh_hip_tlr_apply(const tlr_tile *d_tiles, int ntiles,
         const tlr_chunk *d_chunks, int nchunks, const tlr_entry *d_entries,

@@ -41,6 +41,15 @@ typedef struct
 
 /* Y = alpha * M * X + beta * Y with the plan in device memory (d_tmp holds
  * the coefficients, sum over tiles of 2 r nrhs doubles). */
+/* Static device scratch of the allocation-free product: its size in
+ * doubles, the devices it is set up for, the per-device set-up (called by
+ * starsh_hip_init with the device current) and the lookup of its device
+ * address on a set-up device. */
+#define TLR_SCRATCH_DOUBLES ((int64_t)1 << 22)
+#define TLR_MAX_DEVICES 64
+int tlr_scratch_setup(int device);
+int tlr_scratch(int device, double **ptr);
+
 int starsh_hip_tlr_apply(const tlr_tile *d_tiles, int ntiles,
         const tlr_chunk *d_chunks, int nchunks, const tlr_entry *d_entries,
         const double *d_fac, double *d_tmp, int nrhs, double alpha,
