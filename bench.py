@@ -257,6 +257,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="plain,mixed")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the FP64 rate probe and the product timing "
+                         "(short runs under a profiler)")
     ap.add_argument("--first-bucket", type=int, default=None,
                     help="force the mixed form's first rank bucket (index)")
     ap.add_argument("--set", action="append", default=[],
@@ -310,12 +313,13 @@ def main():
     # attainable FP64 rates of this GPU (MFMA chains, vector FMA)
     mf, vf = ctypes.c_double(0.0), ctypes.c_double(0.0)
     probe = None
-    if S.lib().starsh_hip_fp64_probe(ctypes.byref(mf), ctypes.byref(vf)) == 0:
+    if not args.no_extras and S.lib().starsh_hip_fp64_probe(
+            ctypes.byref(mf), ctypes.byref(vf)) == 0:
         probe = {"mfma": round(mf.value, 2), "valu": round(vf.value, 2)}
     # product with the compressed matrix (starsh_hip_tlr_*, factors
     # resident in HBM): one right-hand side, 20 products
     matvec = None
-    if w["kind"] != "batch":
+    if w["kind"] != "batch" and not args.no_extras:
         sec, nbytes = head["problems"][0].tlr_bench(1, 20)
         if sec > 0:
             matvec = {"form": head_name, "nrhs": 1,

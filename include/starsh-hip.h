@@ -245,6 +245,18 @@ int starsh_hip_generate_tiles(const starsh_hip_kernel_desc *kd,
         const int64_t *r0, const int64_t *c0, const int *m, const int *n,
         const int64_t *off, double *d_out);
 
+/* Morton (Z-order) sort of n points on the GPU (no counterpart in the
+ * reference's device code; the host sort is starsh_particles_zsort_inplace,
+ * src/applications/particles.c:1090-1221): d_x holds ndim x n coordinates
+ * (SoA, ndim <= 4, n < 2^31), d_y receives them in the reference's Morton
+ * order and d_order (optional, int64) the permutation, d_y[., i] =
+ * d_x[., d_order[i]].  The order equals the reference's, ties included:
+ * points of equal key keep their input order when the key has an even
+ * number of 1 bits and reverse it when odd, as the reference's partition
+ * (1-bit points moved to the back in reverse) does. */
+int starsh_hip_zsort(const double *d_x, int64_t n, int ndim, double *d_y,
+        int64_t *d_order);
+
 /* Raw device memory helpers used by the drivers and by tests. */
 int starsh_hip_malloc(void **d_ptr, size_t bytes);
 int starsh_hip_free(void *d_ptr);

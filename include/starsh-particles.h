@@ -84,6 +84,9 @@ int starsh_particles_write_to_file_pointer_binary(const STARSH_particles *data,
         FILE *fp);
 /* particles.c:1090-1221: Morton (Z-order) sort in place. */
 int starsh_particles_zsort_inplace(STARSH_particles *data);
+/* The same sort on the GPU (no counterpart in the reference): identical
+ * permutation, computed by starsh_hip_zsort. */
+int starsh_particles_zsort_inplace_hip(STARSH_particles *data);
 
 #ifdef __cplusplus
 }

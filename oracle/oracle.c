@@ -13,7 +13,10 @@
  *                         dlarnv.  Pinned by tests/golden/omega_*.
  *   oracle_kernel_block   src/applications/spatial/kernel_exp.c:31-110,
  *                         kernel_sqrexp.c:31-111, kernel_matern.c:33-120 (nu in
- *                         {1/2,3/2,5/2} in closed form), kernel_matern2.c,
+ *                         {1/2,3/2,5/2} in closed form; other nu > 0 by the
+ *                         reference's sigma 2^(1-nu)/Gamma(nu) x^nu K_nu(x)
+ *                         with K_nu from its integral representation,
+ *                         oracle_besselk), kernel_matern2.c,
  *                         electrostatics/kernel_coulomb_potential.c:31-99,
  *                         electrodynamics/kernel_sin.c:31-105, kernel_cos.c.
  *   oracle_dsvfr          src/backends/sequential/dense/dsvfr.c:16-47.
@@ -147,6 +150,39 @@ static double matern_closed(double nu, double x)
     return (1.0+x+x*x/3.0)*exp(-x);
 }
 
+/* K_nu(x) = int_0^inf exp(-x cosh t) cosh(nu t) dt (Watson 6.22 (5)) by
+ * the trapezoidal rule with step min(1/20, 1/(2 sqrt x)) -- fine enough
+ * for the Gaussian-like peak of width 1/sqrt(x) at large x -- and
+ * compensated summation, until x cosh t exceeds 800.  A method of its own,
+ * independent of the product's (continued fraction for x >= 2, a coarser
+ * rule below); pinned against scipy.special.kv values in
+ * tests/golden/kernel_golden.json. */
+double oracle_besselk(double nu, double x)
+{
+    const double h = fmin(0.05, 0.5/sqrt(x));
+    double s = 0.5*exp(-x), comp = 0.0;
+    for(int64_t j = 1; ; j++)
+    {
+        const double t = j*h;
+        const double xc = x*cosh(t);
+        if(xc > 800.0)
+            break;
+        const double y = exp(-xc)*cosh(nu*t)-comp;
+        const double u = s+y;
+        comp = (u-s)-y;
+        s = u;
+    }
+    return h*s;
+}
+
+static double matern_any(double sigma, double nu, double x)
+{
+    if(nu == 0.5 || nu == 1.5 || nu == 2.5)
+        return sigma*matern_closed(nu, x);
+    return sigma*pow(2.0, 1.0-nu)/tgamma(nu)*pow(x, nu)*
+        oracle_besselk(nu, x);
+}
+
 void oracle_kernel_block(const oracle_kernel *kd, const double *x1,
         int64_t c1, const double *x2, int64_t c2, int nrows, int ncols,
         const int64_t *irow, const int64_t *icol, double *out, int ld)
@@ -176,12 +212,12 @@ void oracle_kernel_block(const oracle_kernel *kd, const double *x1,
                 case 3:
                     dist = sqrt(dist)*theta;
                     v = dist == 0 ? kd->sigma+kd->noise :
-                        kd->sigma*matern_closed(kd->nu, dist);
+                        matern_any(kd->sigma, kd->nu, dist);
                     break;
                 case 4:
                     dist = sqrt(dist)/kd->beta;
                     v = dist == 0 ? kd->sigma+kd->noise :
-                        kd->sigma*matern_closed(kd->nu, dist);
+                        matern_any(kd->sigma, kd->nu, dist);
                     break;
                 case 5:
                     v = dist == 0 ? 0.0 : 1.0/sqrt(dist);

@@ -6,7 +6,9 @@
  * accumulated over dimensions from 0.0, the exponent is formed with the same
  * divisions, and r == 0 takes the special value.  The Matern kernels use the
  * closed forms of 2^(1-nu)/Gamma(nu) x^nu K_nu(x) for nu = 1/2, 3/2, 5/2:
- * exp(-x), (1+x) exp(-x) and (1+x+x^2/3) exp(-x).
+ * exp(-x), (1+x) exp(-x) and (1+x+x^2/3) exp(-x); any other nu > 0 takes
+ * the reference's formula sigma 2^(1-nu)/Gamma(nu) x^nu K_nu(x)
+ * (kernel_matern.c:115-117) with the K_nu of besselk.h.
  *
  * These host functions serve the API (starsh_problem_get_block, user code);
  * the HIP drivers never call them: they map the pointer to a device
@@ -16,6 +18,7 @@
 #include "starsh-spatial.h"
 #include "starsh-electrostatics.h"
 #include "starsh-electrodynamics.h"
+#include "besselk.h"
 
 #pragma GCC optimize ("no-fast-math")
 
@@ -36,6 +39,8 @@ static void eval_block(int family, int ndim, const double *x1,
     double beta_exp = -kd->beta;
     double beta_sqr = -2*kd->beta*kd->beta;
     double theta = sqrt(2*kd->nu)/kd->beta;
+    const int closed = starsh_matern_closed_nu(kd->nu);
+    const double mcoef = starsh_matern_coef(kd->sigma, kd->nu);
     for(int j = 0; j < ncols; j++)
         for(int i = 0; i < nrows; i++)
         {
@@ -58,13 +63,15 @@ static void eval_block(int family, int ndim, const double *x1,
                     break;
                 case STARSH_HIP_MATERN:
                     dist = sqrt(dist)*theta;
-                    v = dist == 0 ? kd->sigma+kd->noise :
-                        kd->sigma*matern_closed(kd->nu, dist);
+                    v = dist == 0 ? kd->sigma+kd->noise : closed ?
+                        kd->sigma*matern_closed(kd->nu, dist) :
+                        starsh_matern_general(mcoef, kd->nu, dist);
                     break;
                 case STARSH_HIP_MATERN2:
                     dist = sqrt(dist)/kd->beta;
-                    v = dist == 0 ? kd->sigma+kd->noise :
-                        kd->sigma*matern_closed(kd->nu, dist);
+                    v = dist == 0 ? kd->sigma+kd->noise : closed ?
+                        kd->sigma*matern_closed(kd->nu, dist) :
+                        starsh_matern_general(mcoef, kd->nu, dist);
                     break;
                 case STARSH_HIP_COULOMB:
                     v = dist == 0 ? 0.0 : 1.0/sqrt(dist);
@@ -219,7 +226,7 @@ int starsh_kernel_describe(STARSH_kernel *kernel, void *data,
     if(kd->ndim < 1 || kd->ndim > 4 || kd->ndim > data_ndim)
         return -1;
     if((family == STARSH_HIP_MATERN || family == STARSH_HIP_MATERN2) &&
-            kd->nu != 0.5 && kd->nu != 1.5 && kd->nu != 2.5)
+            !(kd->nu > 0))
         return -1;
     return 0;
 }

@@ -518,3 +518,41 @@ int starsh_particles_write_to_file(const STARSH_particles *data,
     fclose(fp);
     return info;
 }
+
+/* starsh_particles_zsort_inplace on the GPU: the same permutation (the
+ * reference's partition order, particles.c:1090-1221), computed by
+ * starsh_hip_zsort on the device.  data->point is replaced by a new
+ * malloc'd array in Morton order, as in the reference (:1150-1160). */
+int starsh_particles_zsort_inplace_hip(STARSH_particles *data)
+{
+    if(data == NULL || data->ndim < 1 || data->count < 0)
+        return STARSH_WRONG_PARAMETER;
+    const STARSH_int n = data->count;
+    const int ndim = data->ndim;
+    if(n == 0)
+        return STARSH_SUCCESS;
+    const size_t bytes = sizeof(double)*(size_t)n*ndim;
+    double *d_x = NULL, *d_y = NULL;
+    double *y = malloc(bytes);
+    if(y == NULL)
+        return STARSH_MALLOC_ERROR;
+    int info = starsh_hip_malloc((void **)&d_x, bytes);
+    if(info == STARSH_SUCCESS)
+        info = starsh_hip_malloc((void **)&d_y, bytes);
+    if(info == STARSH_SUCCESS)
+        info = starsh_hip_memcpy_h2d(d_x, data->point, bytes);
+    if(info == STARSH_SUCCESS)
+        info = starsh_hip_zsort(d_x, n, ndim, d_y, NULL);
+    if(info == STARSH_SUCCESS)
+        info = starsh_hip_memcpy_d2h(y, d_y, bytes);
+    starsh_hip_free(d_x);
+    starsh_hip_free(d_y);
+    if(info != STARSH_SUCCESS)
+    {
+        free(y);
+        return info;
+    }
+    free(data->point);
+    data->point = y;
+    return STARSH_SUCCESS;
+}

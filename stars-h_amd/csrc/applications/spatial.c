@@ -148,9 +148,11 @@ int starsh_ssdata_get_kernel(STARSH_kernel **kernel, STARSH_ssdata *data,
             type == STARSH_SPATIAL_MATERN2 ||
             type == STARSH_SPATIAL_MATERN2_SIMD)
     {
-        if(data->nu != 0.5 && data->nu != 1.5 && data->nu != 2.5)
+        /* nu = 0 makes 2^(1-nu)/Gamma(nu) vanish against an infinite
+         * K_0 x^0 at every distance; the reference fails there too */
+        if(!(data->nu > 0))
         {
-            STARSH_ERROR("Matern kernels support nu in {0.5, 1.5, 2.5}");
+            STARSH_ERROR("Matern kernels need nu > 0");
             return STARSH_WRONG_PARAMETER;
         }
     }

@@ -20,6 +20,12 @@
 #include "hip_driver.h"
 
 static __thread starsh_hip_stats last_stats;
+/* Batches per pipeline run (besides the split by the workspace budget):
+ * 0 or 1 runs all tiles of a run together (default), n > 1 splits them into
+ * n batches, so that the copy-back of one overlaps the compression of the
+ * next.  At C2 (820 tiles) the split costs more device time (9.9 ms in one
+ * batch, 14.1 in two, 15.4 in three) than the overlap saves. */
+static int pipeline_batches = 0;
 static size_t workspace_budget = (size_t)96 << 30;
 
 int starsh_hip_get_stats(starsh_hip_stats *stats)
@@ -58,6 +64,13 @@ int starsh_hip_set_option(const char *name, double value)
         return starsh_hip_proj_int8((int)value);
     if(!strcmp(name, "jacobi_max_sweeps"))
         return starsh_hip_jacobi_sweeps((int)value);
+    if(!strcmp(name, "pipeline_batches"))
+    {
+        if(value < 0 || value > 64)
+            return STARSH_WRONG_PARAMETER;
+        pipeline_batches = (int)value;
+        return STARSH_SUCCESS;
+    }
     if(!strcmp(name, "inject_failure"))
         return hip_driver_inject_failure((int)value);
     if(!strcmp(name, "release_workspace"))
@@ -71,6 +84,12 @@ int starsh_hip_set_option(const char *name, double value)
 size_t hip_driver_workspace_budget(void)
 {
     return workspace_budget;
+}
+
+STARSH_int hip_driver_chunk_tiles(STARSH_int ntiles)
+{
+    const STARSH_int nb = pipeline_batches > 1 ? pipeline_batches : 1;
+    return (ntiles+nb-1)/nb;
 }
 
 double hip_driver_now(void)
@@ -200,15 +219,42 @@ void hip_driver_release(hip_problem *hp)
     hp->d_xr = hp->d_xc = NULL;
 }
 
-/* Step 1 (drsdd.c:53-90), deferred: the reference allocates U and V
- * maxrank columns wide per tile before compressing.  Here the factors stay
- * on the device until every tile is done, and hip_driver_download then
- * allocates host buffers of the exact sizes (U_t m x r, V_t n x r, packed).
- * hf owns what it points to; on failure hip_driver_free_factors releases
- * it. */
+/* Host buffer for factors: 2 MiB aligned and, when large, backed by huge
+ * pages where the kernel allows it, so that first touches fault 2 MiB at a
+ * time.  Released with free(), as the reference's malloc'd buffers.  Only
+ * the pages written are ever backed by memory. */
+static double *alloc_host(size_t n)
+{
+    void *p = NULL;
+    const size_t bytes = sizeof(double)*(n > 0 ? n : 1);
+    if(posix_memalign(&p, (size_t)2 << 20, bytes) != 0)
+        return NULL;
+    if(bytes >= ((size_t)4 << 20))
+        (void)madvise(p, bytes, MADV_HUGEPAGE);
+    return p;
+}
+
+/* Sketch width of the plain form for a far-field tile. */
+static int full_width(const STARSH_blrf *F, STARSH_int bi, int maxrank,
+        int oversample)
+{
+    const STARSH_int m = F->row_cluster->size[F->block_far[2*bi]];
+    const STARSH_int n = F->col_cluster->size[F->block_far[2*bi+1]];
+    STARSH_int k = (STARSH_int)maxrank+oversample;
+    k = k < m ? k : m;
+    return (int)(k < n ? k : n);
+}
+
+/* Step 1 (drsdd.c:53-90): host buffers for U and V of every far-field
+ * tile.  The reference gives each tile maxrank columns at a fixed offset;
+ * here each finished tile takes the next m x r (n x r) block of alloc_U
+ * (alloc_V) in the order tiles finish, so the factors are contiguous and
+ * the copy from the device writes only what it needs.  The buffers are
+ * sized for the largest possible ranks, min(maxrank, k) per tile; pages
+ * never written cost nothing.  hf owns what it points to; on failure
+ * hip_driver_free_factors releases it. */
 int hip_driver_alloc_factors(STARSH_blrf *F, int maxrank, hip_factors *hf)
 {
-    (void)maxrank;
     memset(hf, 0, sizeof(*hf));
     STARSH_int nfar = F->nblocks_far;
     if(nfar == 0)
@@ -221,125 +267,73 @@ int hip_driver_alloc_factors(STARSH_blrf *F, int maxrank, hip_factors *hf)
     STARSH_MALLOC(hf->voff, nfar);
     if(hf->far_U == NULL || hf->far_V == NULL)
         return STARSH_MALLOC_ERROR;
+    const int os = starsh_params.oversample > 0 ? starsh_params.oversample :
+        10;
+    size_t su = 0, sv = 0;
     for(STARSH_int bi = 0; bi < nfar; bi++)
     {
         hf->far_rank[bi] = 0;
         hf->uoff[bi] = hf->voff[bi] = -1;
+        int r = full_width(F, bi, maxrank, os);
+        r = r < maxrank ? r : maxrank;
+        su += (size_t)F->row_cluster->size[F->block_far[2*bi]]*r;
+        sv += (size_t)F->col_cluster->size[F->block_far[2*bi+1]]*r;
+    }
+    hf->ucap = (int64_t)su;
+    hf->vcap = (int64_t)sv;
+    hf->alloc_U = alloc_host(su);
+    hf->alloc_V = alloc_host(sv);
+    if(hf->alloc_U == NULL || hf->alloc_V == NULL)
+    {
+        STARSH_ERROR("malloc() failed");
+        return STARSH_MALLOC_ERROR;
     }
     return STARSH_SUCCESS;
 }
 
-/* Host buffer for factors: 2 MiB aligned and, when large, backed by huge
- * pages where the kernel allows it, so that first touches fault 2 MiB at a
- * time.  Released with free(), as the reference's malloc'd buffers. */
-static double *alloc_host(size_t n)
+/* Doubles and segments the factors of a batch of problems can take at
+ * most (the device arena of the copy-back). */
+void hip_driver_factor_bound(const hip_batch *B, int64_t *doubles,
+        int64_t *segments)
 {
-    void *p = NULL;
-    const size_t bytes = sizeof(double)*(n > 0 ? n : 1);
-    if(posix_memalign(&p, (size_t)2 << 20, bytes) != 0)
-        return NULL;
-    if(bytes >= ((size_t)4 << 20))
-        (void)madvise(p, bytes, MADV_HUGEPAGE);
-    return p;
-}
-
-typedef struct
-{
-    int64_t aoff, len;
-    double *dst;
-} seg_t;
-
-static int seg_cmp(const void *a, const void *b)
-{
-    const int64_t x = ((const seg_t *)a)->aoff, y = ((const seg_t *)b)->aoff;
-    return x < y ? -1 : x > y;
-}
-
-/* Step 1 done late, and the copy-back: host buffers of the exact sizes for
- * every problem of the batch, one download of the device arena into them,
- * and the U/V wrappers (shape m x r, n x r). */
-int hip_driver_download(const hip_batch *B, starsh_hip_stats *st)
-{
-    int64_t nseg = 0;
+    *doubles = 0;
+    *segments = 0;
     for(int q = 0; q < B->nprob; q++)
-        nseg += 2*B->hf[q].nfar;
-    seg_t *seg = malloc(sizeof(seg_t)*(nseg+1));
-    if(seg == NULL)
-        return STARSH_MALLOC_ERROR;
-    nseg = 0;
+    {
+        *doubles += B->hf[q].ucap+B->hf[q].vcap;
+        *segments += 2*B->hf[q].nfar;
+    }
+}
+
+/* The U/V wrappers (shape m x r, n x r) of every tile once all copies have
+ * landed; false far-field tiles (rank -1) get none. */
+int hip_driver_wrap(const hip_batch *B)
+{
     for(int q = 0; q < B->nprob; q++)
     {
         hip_factors *hf = B->hf+q;
         const STARSH_blrf *F = B->F[q];
-        const STARSH_cluster *RC = F->row_cluster, *CC = F->col_cluster;
-        if(hf->nfar == 0)
-            continue;
-        size_t su = 0, sv = 0;
-        for(STARSH_int bi = 0; bi < hf->nfar; bi++)
-        {
-            const int r = hf->far_rank[bi];
-            if(r <= 0)
-                continue;
-            su += (size_t)RC->size[F->block_far[2*bi]]*r;
-            sv += (size_t)CC->size[F->block_far[2*bi+1]]*r;
-        }
-        hf->alloc_U = alloc_host(su);
-        hf->alloc_V = alloc_host(sv);
-        if(hf->alloc_U == NULL || hf->alloc_V == NULL)
-        {
-            free(seg);
-            STARSH_ERROR("malloc() failed");
-            return STARSH_MALLOC_ERROR;
-        }
-        size_t ou = 0, ov = 0;
         for(STARSH_int bi = 0; bi < hf->nfar; bi++)
         {
             const int r = hf->far_rank[bi];
             if(r < 0)
                 continue;
-            const int m = (int)RC->size[F->block_far[2*bi]];
-            const int n = (int)CC->size[F->block_far[2*bi+1]];
+            const int m = (int)F->row_cluster->size[F->block_far[2*bi]];
+            const int n = (int)F->col_cluster->size[F->block_far[2*bi+1]];
             int shape_U[2] = {m, r}, shape_V[2] = {n, r};
+            /* rank-0 tiles point at the start of the buffers */
+            double *u = hf->alloc_U+(hf->uoff[bi] >= 0 ? hf->uoff[bi] : 0);
+            double *v = hf->alloc_V+(hf->voff[bi] >= 0 ? hf->voff[bi] : 0);
             int info = array_from_buffer(hf->far_U+bi, 2, shape_U, 'd', 'F',
-                    hf->alloc_U+ou);
+                    u);
             if(info == STARSH_SUCCESS)
                 info = array_from_buffer(hf->far_V+bi, 2, shape_V, 'd', 'F',
-                        hf->alloc_V+ov);
+                        v);
             if(info != STARSH_SUCCESS)
-            {
-                free(seg);
                 return info;
-            }
-            if(r > 0)
-            {
-                seg[nseg++] = (seg_t){hf->uoff[bi], (int64_t)m*r,
-                    hf->alloc_U+ou};
-                seg[nseg++] = (seg_t){hf->voff[bi], (int64_t)n*r,
-                    hf->alloc_V+ov};
-            }
-            ou += (size_t)m*r;
-            ov += (size_t)n*r;
         }
     }
-    qsort(seg, nseg, sizeof(seg_t), seg_cmp);
-    int64_t *aoff = malloc(sizeof(int64_t)*(nseg+1));
-    int64_t *len = malloc(sizeof(int64_t)*(nseg+1));
-    double **dst = malloc(sizeof(double *)*(nseg+1));
-    int info = aoff && len && dst ? STARSH_SUCCESS : STARSH_MALLOC_ERROR;
-    for(int64_t q = 0; info == STARSH_SUCCESS && q < nseg; q++)
-    {
-        aoff[q] = seg[q].aoff;
-        len[q] = seg[q].len;
-        dst[q] = seg[q].dst;
-    }
-    if(info == STARSH_SUCCESS)
-        info = rsdd_arena_download(nseg, aoff, len, dst, &st->d2h_seconds,
-                &st->scatter_seconds);
-    free(seg);
-    free(aoff);
-    free(len);
-    free(dst);
-    return info;
+    return STARSH_SUCCESS;
 }
 
 /* Releases what hf holds.  The wrappers only view alloc_U/alloc_V, so
@@ -624,11 +618,11 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
     int64_t *uo = malloc(sizeof(int64_t)*ntiles);
     int64_t *vo = malloc(sizeof(int64_t)*ntiles);
     RsddProblem *probs = malloc(sizeof(RsddProblem)*B->nprob);
-    int64_t *au = malloc(sizeof(int64_t)*ntiles);
-    int64_t *av = malloc(sizeof(int64_t)*ntiles);
+    double **du = malloc(sizeof(double *)*ntiles);
+    double **dv = malloc(sizeof(double *)*ntiles);
     int64_t *so = malloc(sizeof(int64_t)*ntiles);
-    if(!m || !n || !pp || !kc || !r0 || !c0 || !uo || !vo || !probs || !au ||
-            !av || !so)
+    if(!m || !n || !pp || !kc || !r0 || !c0 || !uo || !vo || !probs || !du ||
+            !dv || !so)
         info = STARSH_MALLOC_ERROR;
     for(STARSH_int t = 0; info == STARSH_SUCCESS && t < ntiles; t++)
         so[t] = (int64_t)t*sstride;
@@ -658,12 +652,15 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
             STARSH_WARNING("This was only tested on square tiles, error of "
                     "approximation may be much higher, than demanded");
     }
+    /* batches of at most chunk_tiles tiles, so that the copy-back of one
+     * overlaps the compression of the next */
+    const STARSH_int chunk_tiles = hip_driver_chunk_tiles(ntiles);
     STARSH_int b0 = 0;
     while(info == STARSH_SUCCESS && b0 < ntiles)
     {
         size_t bytes = 0;
         STARSH_int b1 = b0;
-        while(b1 < ntiles)
+        while(b1 < ntiles && (chunk_tiles <= 0 || b1-b0 < chunk_tiles))
         {
             const int q = pidx[b1];
             const int mr = B->maxrank[q], os = B->hp[q].oversample;
@@ -724,25 +721,41 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
             accepted[t] = accept ? accept(rank_out[t], kk, kfull,
                     flags_out[t], B->hp[q].oversample) : 1;
         }
-        /* factors of the finished tiles to the device arena; the host
-         * copy happens once all tiles are done (hip_driver_download) */
-        info = rsdd_arena_append(&out, (int)(b1-b0), m+b0, n+b0, accepted+b0,
-                au+b0, av+b0);
-        for(STARSH_int t = b0; info == STARSH_SUCCESS && t < b1; t++)
+        /* finished tiles take the next blocks of their problem's host
+         * buffers; their factors are queued for the copy-back, which runs
+         * while the next batch computes */
+        for(STARSH_int t = b0; t < b1; t++)
         {
+            du[t] = dv[t] = NULL;
             if(!accepted[t])
                 continue;
             hip_factors *hf = B->hf+pidx[t];
             const STARSH_int bi = idx[t];
-            hf->far_rank[bi] = rank_out[t];
-            hf->uoff[bi] = au[t];
-            hf->voff[bi] = av[t];
+            const int r = rank_out[t];
+            hf->far_rank[bi] = r;
+            if(r <= 0)
+                continue;
+            if(hf->uused+(int64_t)m[t]*r > hf->ucap ||
+                    hf->vused+(int64_t)n[t]*r > hf->vcap)
+            {
+                info = STARSH_UNKNOWN_ERROR;
+                break;
+            }
+            hf->uoff[bi] = hf->uused;
+            hf->voff[bi] = hf->vused;
+            hf->uused += (int64_t)m[t]*r;
+            hf->vused += (int64_t)n[t]*r;
+            du[t] = hf->alloc_U+hf->uoff[bi];
+            dv[t] = hf->alloc_V+hf->voff[bi];
         }
+        if(info == STARSH_SUCCESS)
+            info = rsdd_out_append(&out, (int)(b1-b0), m+b0, n+b0,
+                    accepted+b0, du+b0, dv+b0);
         st->copy_seconds += hip_driver_now()-tc;
         b0 = b1;
     }
     free(m); free(n); free(pp); free(kc); free(r0); free(c0); free(uo);
-    free(vo); free(probs); free(au); free(av); free(so);
+    free(vo); free(probs); free(du); free(dv); free(so);
     return info;
 }
 
@@ -821,8 +834,6 @@ int hip_compress(int mixed, int nprob, STARSH_blrm **matrix,
     memset(&st, 0, sizeof(st));
     int info = B.hp && B.hf ? STARSH_SUCCESS : STARSH_MALLOC_ERROR;
     int nprep = 0;
-    if(info == STARSH_SUCCESS)
-        info = rsdd_arena_reset();
     for(int q = 0; info == STARSH_SUCCESS && q < nprob; q++)
     {
         info = hip_driver_prepare(format[q], maxrank[q], tol[q], onfly,
@@ -831,6 +842,14 @@ int hip_compress(int mixed, int nprob, STARSH_blrm **matrix,
             break;
         nprep++;
         info = hip_driver_alloc_factors(format[q], maxrank[q], B.hf+q);
+    }
+    int out_running = 0;
+    if(info == STARSH_SUCCESS)
+    {
+        int64_t nd, ns;
+        hip_driver_factor_bound(&B, &nd, &ns);
+        info = rsdd_out_begin(nd, ns);
+        out_running = info == STARSH_SUCCESS;
     }
     st.prepare_seconds = hip_driver_now()-t_start;
     /* all far-field tiles, grouped by kernel family and dimension */
@@ -862,12 +881,18 @@ int hip_compress(int mixed, int nprob, STARSH_blrm **matrix,
             hip_plain_tiles(&B, nt, pidx, idx, &st);
     }
     free(pidx); free(idx); free(done);
-    if(info == STARSH_SUCCESS)
+    if(out_running)
     {
+        /* the copy threads must be done with the host buffers before
+         * anything else touches or frees them, also after an error */
         double td = hip_driver_now();
-        info = hip_driver_download(&B, &st);
+        int e = rsdd_out_end(&st.d2h_seconds, &st.scatter_seconds);
         st.copy_seconds += hip_driver_now()-td;
+        if(info == STARSH_SUCCESS)
+            info = e;
     }
+    if(info == STARSH_SUCCESS)
+        info = hip_driver_wrap(&B);
     if(info == STARSH_SUCCESS)
         hip_driver_rank_stats(&B, &st);
     double tf = hip_driver_now();

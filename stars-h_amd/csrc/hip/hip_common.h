@@ -6,6 +6,7 @@
 #include <stdint.h>
 #include "starsh.h"
 #include "starsh-hip.h"
+#include "besselk.h"
 
 #define HIP_TRY(expr) do { hipError_t e_ = (expr); if(e_ != hipSuccess) { \
     fprintf(stderr, "STARSH HIP ERROR: %s at %s:%d\n", hipGetErrorString(e_), \
@@ -31,6 +32,8 @@ struct KernelParams
     double ibexp;  // 1/bexp, 1/bsqr, 1/beta (kval)
     double ibsqr;
     double ibeta;
+    double mcoef;  // sigma 2^(1-nu)/Gamma(nu) (Matern with general nu)
+    int closed;    // Matern nu in {1/2, 3/2, 5/2}: closed forms
 };
 
 static inline KernelParams make_params(const starsh_hip_kernel_desc *kd)
@@ -47,6 +50,8 @@ static inline KernelParams make_params(const starsh_hip_kernel_desc *kd)
     p.noise = kd->noise;
     p.wave_k = kd->wave_k;
     p.diag = kd->diag;
+    p.mcoef = starsh_matern_coef(kd->sigma, kd->nu);
+    p.closed = starsh_matern_closed_nu(kd->nu);
     p.ibexp = 1.0/p.bexp;
     p.ibsqr = 1.0/p.bsqr;
     p.ibeta = 1.0/p.beta;
@@ -100,6 +105,14 @@ __device__ inline double matern_closed(double nu, double x)
     return (1.0+x+x*x/3.0)*e;
 }
 
+// Matern value at scaled distance x > 0: closed forms for nu = 1/2, 3/2,
+// 5/2, else sigma 2^(1-nu)/Gamma(nu) x^nu K_nu(x) (besselk.h).
+__device__ inline double matern_value(const KernelParams &kp, double x)
+{
+    return kp.closed ? kp.sigma*matern_closed(kp.nu, x) :
+        starsh_matern_general(kp.mcoef, kp.nu, x);
+}
+
 // Entry A(i, j) = f(|x_i - y_j|) in FP64 with the reference's operation
 // order (no contraction into fused multiply-adds).
 template <int NDIM>
@@ -128,13 +141,11 @@ __device__ inline double kernel_value(const KernelParams &kp,
             break;
         case STARSH_HIP_MATERN:
             dist = sqrt(dist)*kp.theta;
-            v = dist == 0 ? kp.sigma+kp.noise :
-                kp.sigma*matern_closed(kp.nu, dist);
+            v = dist == 0 ? kp.sigma+kp.noise : matern_value(kp, dist);
             break;
         case STARSH_HIP_MATERN2:
             dist = sqrt(dist)/kp.beta;
-            v = dist == 0 ? kp.sigma+kp.noise :
-                kp.sigma*matern_closed(kp.nu, dist);
+            v = dist == 0 ? kp.sigma+kp.noise : matern_value(kp, dist);
             break;
         case STARSH_HIP_COULOMB:
             v = dist == 0 ? 0.0 : 1.0/sqrt(dist);
@@ -180,10 +191,10 @@ __device__ inline double kval_of_dist(const KernelParams &kp, double dist)
                 kp.sigma*exp(dist*kp.ibsqr);
         case STARSH_HIP_MATERN:
             return dist == 0 ? kp.sigma+kp.noise :
-                kp.sigma*matern_closed(kp.nu, sqrt(dist)*kp.theta);
+                matern_value(kp, sqrt(dist)*kp.theta);
         case STARSH_HIP_MATERN2:
             return dist == 0 ? kp.sigma+kp.noise :
-                kp.sigma*matern_closed(kp.nu, sqrt(dist)*kp.ibeta);
+                matern_value(kp, sqrt(dist)*kp.ibeta);
         case STARSH_HIP_COULOMB:
             return dist == 0 ? 0.0 : rsqrt(dist);
         case STARSH_HIP_SIN:
@@ -306,6 +317,11 @@ __device__ inline double kval_scaled(const KernelParams &kp, double d,
         case STARSH_HIP_MATERN2:
         {
             const double x = sqrt(d);
+            if(!kp.closed)
+                // general nu: no table, and the formula is singular at
+                // d = 0, so the select is always made
+                return d == 0 ? kp.sigma+kp.noise :
+                    starsh_matern_general(kp.mcoef, kp.nu, x);
             const double e = sexp_nonpos(-x, tab);
             // 1, 1 + x or 1 + x + x^2/3 for nu = 0.5, 1.5, 2.5, as one
             // polynomial whose coefficients do not depend on the entry

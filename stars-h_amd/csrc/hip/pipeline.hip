@@ -519,183 +519,6 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
 }
 
 
-// Arena of the factors of one API call (device, grow-only, kept between
-// calls) and its fill level in doubles.
-static Pool arena_pool;
-static int64_t arena_used = 0;
-static Pool seg_pool;
-
-int rsdd_arena_reset(void)
-{
-    arena_used = 0;
-    return STARSH_SUCCESS;
-}
-
-int rsdd_arena_append(const RsddOutput *out, int ntiles, const int *m,
-        const int *n, const int *take, int64_t *uo, int64_t *vo)
-{
-    hipStream_t s = starsh_hip_stream();
-    std::vector<int64_t> seg;
-    int64_t add = 0;
-    for(int t = 0; t < ntiles; t++)
-    {
-        uo[t] = vo[t] = -1;
-        if(!take[t] || out->rank[t] <= 0)
-            continue;
-        const int64_t lu = (int64_t)m[t]*out->rank[t];
-        const int64_t lv = (int64_t)n[t]*out->rank[t];
-        uo[t] = arena_used+add;
-        seg.insert(seg.end(), {out->uoff[t], uo[t], lu});
-        add += lu;
-        vo[t] = arena_used+add;
-        seg.insert(seg.end(), {out->voff[t], vo[t], lv});
-        add += lv;
-    }
-    const int nseg = (int)(seg.size()/3);
-    if(nseg == 0)
-        return STARSH_SUCCESS;
-    const size_t need = sizeof(double)*(size_t)(arena_used+add);
-    if(need > arena_pool.bytes)
-    {
-        // grow: a new buffer with the old contents copied over
-        Pool grown;
-        TRY(grown.reserve(std::max(need, arena_pool.bytes+arena_pool.bytes/2)));
-        if(arena_used > 0)
-            HIP_TRY(hipMemcpyAsync(grown.p, arena_pool.p,
-                        sizeof(double)*(size_t)arena_used,
-                        hipMemcpyDeviceToDevice, s));
-        arena_pool.release();
-        arena_pool.p = grown.p;
-        arena_pool.bytes = grown.bytes;
-        grown.p = nullptr;
-        grown.bytes = 0;
-    }
-    TRY(seg_pool.reserve(sizeof(int64_t)*seg.size()));
-    int64_t *d_seg = (int64_t *)seg_pool.p;
-    HIP_TRY(hipMemcpyAsync(d_seg, seg.data(), sizeof(int64_t)*seg.size(),
-                hipMemcpyHostToDevice, s));
-    for(int s0 = 0; s0 < nseg; s0 += 65535)
-    {
-        const int cnt = std::min(nseg-s0, 65535);
-        hipLaunchKernelGGL(pack_kernel, dim3(16, cnt), dim3(256), 0, s,
-                out->d_uv, (double *)arena_pool.p, d_seg+3*(int64_t)s0,
-                cnt);
-        HIP_TRY(hipGetLastError());
-    }
-    // the segment list is host memory of this call: wait until the
-    // transfer has read it
-    HIP_TRY(hipStreamSynchronize(s));
-    arena_used += add;
-    return STARSH_SUCCESS;
-}
-
-// Events of the chunked download, created once.
-static std::vector<hipEvent_t> chunk_events;
-
-int rsdd_arena_download(int64_t nseg, const int64_t *aoff,
-        const int64_t *len, double *const *dst, double *d2h_seconds,
-        double *scatter_seconds)
-{
-    const auto t0 = std::chrono::steady_clock::now();
-    hipStream_t s = starsh_hip_stream();
-    if(nseg <= 0)
-        return STARSH_SUCCESS;
-    const int64_t lo = aoff[0], hi = aoff[nseg-1]+len[nseg-1];
-    const int64_t total = hi-lo;
-    if(total <= 0)
-        return STARSH_SUCCESS;
-    TRY(stage_pool.reserve(sizeof(double)*(size_t)total));
-    double *h = (double *)stage_pool.p;
-    const double *d = (const double *)arena_pool.p+lo;
-    // chunks of 4 Mi doubles (32 MiB)
-    const int64_t CH = (int64_t)1 << 22;
-    const int nch = (int)((total+CH-1)/CH);
-    while((int)chunk_events.size() < nch)
-    {
-        hipEvent_t e;
-        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        chunk_events.push_back(e);
-    }
-    for(int c = 0; c < nch; c++)
-    {
-        const int64_t c0 = (int64_t)c*CH, c1 = std::min(total, c0+CH);
-        HIP_TRY(hipMemcpyAsync(h+c0, d+c0, sizeof(double)*(size_t)(c1-c0),
-                    hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipEventRecord(chunk_events[c], s));
-    }
-    // Host threads: touch their share of the destination pages (the first
-    // touch maps and zeroes them) while the transfer runs, then copy each
-    // chunk as soon as it has arrived, every thread taking 1/nth of it.
-    unsigned nth = std::thread::hardware_concurrency();
-    nth = nth < 1 ? 1 : (nth > 16 ? 16 : nth);
-    if(total < ((int64_t)1 << 18))
-        nth = 1;
-    std::vector<int> fail(nth, 0);
-    auto copy_range = [&](int64_t a, int64_t b) {
-        // segments are in increasing arena order: find the first one
-        // ending after a
-        int64_t q = std::upper_bound(aoff, aoff+nseg, lo+a)-aoff-1;
-        q = q < 0 ? 0 : q;
-        for(; q < nseg && aoff[q]-lo < b; q++)
-        {
-            const int64_t so = aoff[q]-lo;
-            const int64_t x = std::max(a, so), y = std::min(b, so+len[q]);
-            if(x < y)
-                memcpy(dst[q]+(x-so), h+x, sizeof(double)*(size_t)(y-x));
-        }
-    };
-    auto touch_range = [&](int64_t a, int64_t b) {
-        int64_t q = std::upper_bound(aoff, aoff+nseg, lo+a)-aoff-1;
-        q = q < 0 ? 0 : q;
-        for(; q < nseg && aoff[q]-lo < b; q++)
-        {
-            const int64_t so = aoff[q]-lo;
-            const int64_t x = std::max(a, so), y = std::min(b, so+len[q]);
-            volatile char *p = (volatile char *)(dst[q]+(x-so));
-            const int64_t nbytes = sizeof(double)*(y-x);
-            for(int64_t o = 0; o < nbytes; o += 4096)
-                p[o] = 0;
-        }
-    };
-    auto t_last = t0;
-    auto work = [&](unsigned id) {
-        touch_range(total*id/nth, total*(id+1)/nth);
-        for(int c = 0; c < nch; c++)
-        {
-            if(hipEventSynchronize(chunk_events[c]) != hipSuccess)
-            {
-                fail[id] = 1;
-                return;
-            }
-            if(id == 0 && c == nch-1)
-                t_last = std::chrono::steady_clock::now();
-            const int64_t c0 = (int64_t)c*CH, c1 = std::min(total, c0+CH);
-            copy_range(c0+(c1-c0)*id/nth, c0+(c1-c0)*(id+1)/nth);
-        }
-    };
-    std::vector<std::thread> th;
-    for(unsigned id = 1; id < nth; id++)
-        th.emplace_back(work, id);
-    work(0);
-    for(auto &x : th)
-        x.join();
-    const auto t1 = std::chrono::steady_clock::now();
-    for(int f : fail)
-        if(f)
-        {
-            fprintf(stderr, "STARSH HIP ERROR: device-to-host copy of the "
-                    "factors failed\n");
-            return STARSH_UNKNOWN_ERROR;
-        }
-    // d2h: until the last chunk has arrived (the host copies of the
-    // earlier chunks overlap it); scatter: the host work left after that
-    if(t_last < t0)
-        t_last = t0;
-    *d2h_seconds += std::chrono::duration<double>(t_last-t0).count();
-    *scatter_seconds += std::chrono::duration<double>(t1-t_last).count();
-    return STARSH_SUCCESS;
-}
-
 // Frees the buffers kept between calls.
 void rsdd_release_pools(void)
 {
@@ -703,9 +526,7 @@ void rsdd_release_pools(void)
     uv_pool.release();
     pack_pool.release();
     stage_pool.release();
-    arena_pool.release();
-    seg_pool.release();
-    arena_used = 0;
+    rsdd_out_release();
 }
 
 extern "C" int starsh_hip_proj_int8(int on)

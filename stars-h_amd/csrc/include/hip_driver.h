@@ -22,8 +22,10 @@ typedef struct hip_factors
     Array **far_U, **far_V;
     int *far_rank;
     double *alloc_U, *alloc_V;
-    int64_t *uoff, *voff;  /* arena offsets of each tile's U and V (-1:
-                              none) until the download */
+    int64_t *uoff, *voff;  /* offsets of each tile's U and V in alloc_U /
+                              alloc_V (-1: none) */
+    int64_t ucap, vcap;    /* doubles of alloc_U / alloc_V */
+    int64_t uused, vused;  /* doubles taken so far */
     STARSH_int nfar;       /* entries of far_U, far_V and far_rank */
 } hip_factors;
 
@@ -69,9 +71,13 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
         int *rank_out, int *flags_out, int *accepted, hip_accept_fn accept,
         double *S_out, int sstride, starsh_hip_stats *st);
 void hip_driver_rank_stats(const hip_batch *B, starsh_hip_stats *st);
-/* After the last hip_driver_run of a batch: host U/V buffers of the exact
- * sizes for every problem, filled from the device arena, with wrappers. */
-int hip_driver_download(const hip_batch *B, starsh_hip_stats *st);
+/* Bound on the factor doubles and segments of a batch of problems. */
+void hip_driver_factor_bound(const hip_batch *B, int64_t *doubles,
+        int64_t *segments);
+/* U/V wrappers of every tile after the copy-back has finished. */
+int hip_driver_wrap(const hip_batch *B);
+/* Tiles per pipeline batch for a run of ntiles tiles. */
+STARSH_int hip_driver_chunk_tiles(STARSH_int ntiles);
 
 /* The plain and mixed forms over a list of tiles of a batch. */
 int hip_plain_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,

@@ -65,30 +65,28 @@ typedef struct rsdd_output
 
 int rsdd_batch(const RsddInput *in, RsddOutput *out);
 
-/* Device arena of the factors of one API call.  Tiles are compressed in
- * several batches (workspace chunks, rank rounds); the factors of the
- * tiles each batch finishes are appended to the arena, and one download at
- * the end of the call brings them all to the caller's host buffers, when
- * every rank (so every buffer size) is known.  The arena is kept between
- * calls (grow-only).
- *   rsdd_arena_reset   empties it (start of a call);
- *   rsdd_arena_append  appends U_t then V_t of every tile t < ntiles of
- *                      the last rsdd_batch with take[t] != 0 and rank > 0;
- *                      their arena offsets (doubles) go to uo[t], vo[t]
- *                      (-1 for the other tiles);
- *   rsdd_arena_download copies nseg arena segments (offset aoff[q], len[q]
- *                      doubles) to host dst[q]: device-to-host in chunks
- *                      through pinned memory, the copy into dst overlapped
- *                      with the transfer of the next chunk by host threads,
- *                      which first touch the destination pages while the
- *                      transfer starts.  The segments must be in increasing
- *                      arena order. */
-int rsdd_arena_reset(void);
-int rsdd_arena_append(const RsddOutput *out, int ntiles, const int *m,
-        const int *n, const int *take, int64_t *uo, int64_t *vo);
-int rsdd_arena_download(int64_t nseg, const int64_t *aoff,
-        const int64_t *len, double *const *dst, double *d2h_seconds,
-        double *scatter_seconds);
+/* Copy-back of the factors of one API call, overlapped with the
+ * compression of later batches (csrc/hip/copyout.hip).
+ *   rsdd_out_begin   reserves the device arena for the call (at most
+ *                    max_doubles doubles of factors in max_segments
+ *                    segments) and starts the copy threads;
+ *   rsdd_out_append  packs U_t then V_t of every tile t < ntiles of the
+ *                    last rsdd_batch with take[t] != 0 and rank > 0 into
+ *                    the arena and queues their copy to the host buffers
+ *                    dst_u[t] (m x rank) and dst_v[t] (n x rank), which
+ *                    must stay valid until rsdd_out_end; returns without
+ *                    waiting;
+ *   rsdd_out_end     waits for every queued copy and stops the threads;
+ *                    adds the copy window (first queued batch to last
+ *                    copy) and the wait inside this call to the two
+ *                    counters; returns the first error.  Must follow every
+ *                    successful rsdd_out_begin, also on error paths. */
+int rsdd_out_begin(int64_t max_doubles, int64_t max_segments);
+int rsdd_out_append(const RsddOutput *out, int ntiles, const int *m,
+        const int *n, const int *take, double *const *dst_u,
+        double *const *dst_v);
+int rsdd_out_end(double *window_seconds, double *tail_seconds);
+void rsdd_out_release(void);
 
 /* Frees the workspace and staging buffers kept between calls. */
 void rsdd_release_pools(void);

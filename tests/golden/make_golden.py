@@ -9,7 +9,9 @@ omega_golden.npz  LAPACK dlarnv(idist, iseed={0,0,0,1}, n) called in the
 kernel_golden.json  kernel values at chosen distances from closed forms and,
                   for Matern, from scipy.special.kv / gamma (the general
                   2^(1-nu)/Gamma(nu) x^nu K_nu(x) formula the reference
-                  evaluates with GSL).
+                  evaluates with GSL), for nu = 1/2, 3/2, 5/2 and the
+                  general values GENERAL_NU; distances reach scaled
+                  arguments x from 1e-5 to about 100.
 particles.txt     copy of the reference's own fixture testing/particles.txt
                   (36 3-D points) used by its particles_io test.
 """
@@ -23,6 +25,9 @@ import numpy as np
 import scipy.special as sp
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+# Matern smoothness values without a closed form (general nu through
+# scipy.special.kv, the reference's GSL formula)
+GENERAL_NU = (0.3, 0.8, 1.2, 2.2, 3.7, 6.0)
 
 
 def lapack():
@@ -62,7 +67,8 @@ def main():
              digest_vals=np.array(list(digests.values())),
              lapack=np.array(os.path.basename(name)))
     # kernel values: distance r between (0,..,0) and (r,0,..)
-    rs = [0.0, 1e-3, 0.01, 0.05, 0.1, 0.2, 0.5, 1.0, 1.4142135623730951]
+    rs = [0.0, 1e-6, 1e-3, 0.01, 0.05, 0.1, 0.15, 0.2, 0.3, 0.5, 1.0,
+          1.4142135623730951]
     out = []
     for r in rs:
         rec = {"r": r}
@@ -70,7 +76,7 @@ def main():
         rec["exp"] = sigma + noise if r == 0 else sigma * math.exp(-r / beta)
         rec["sqrexp"] = sigma + noise if r == 0 else \
             sigma * math.exp(-r * r / (2 * beta * beta))
-        for nu in (0.5, 1.5, 2.5):
+        for nu in (0.5, 1.5, 2.5) + GENERAL_NU:
             for var, x in (("matern", math.sqrt(2 * nu) * r / beta),
                            ("matern2", r / beta)):
                 rec[f"{var}_{nu}"] = sigma + noise if r == 0 else \

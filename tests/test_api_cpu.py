@@ -143,7 +143,8 @@ def test_tlr_format_structure():
 
 @pytest.mark.parametrize("kernel,kw", [
     ("exp", dict(beta=0.1)), ("sqrexp", dict(beta=0.2)),
-    ("matern", dict(beta=0.1, nu=1.5)), ("matern2", dict(beta=0.1, nu=2.5))])
+    ("matern", dict(beta=0.1, nu=1.5)), ("matern2", dict(beta=0.1, nu=2.5)),
+    ("matern", dict(beta=0.1, nu=0.8)), ("matern2", dict(beta=0.3, nu=4.3))])
 def test_host_spatial_kernels_match_oracle(kernel, kw):
     p = S.TLRProblem("spatial", 900, 3, 300, seed=2, place=1, kernel=kernel,
                      **kw)
@@ -153,7 +154,11 @@ def test_host_spatial_kernels_match_oracle(kernel, kw):
         A = p.host_block(i, j)
         R = O.kernel_block(kd, pts, pts, np.arange(300 * i, 300 * i + 300),
                            np.arange(300 * j, 300 * j + 300))
-        assert np.array_equal(A, R)
+        if kw.get("nu", 0.5) in (0.5, 1.5, 2.5):
+            assert np.array_equal(A, R)
+        else:
+            # general nu: two K_nu methods, each within a few 1e-14
+            assert np.max(np.abs(A - R)) <= 1e-13 * np.max(np.abs(R))
 
 
 @pytest.mark.parametrize("kind", ["coulomb", "sin", "cos"])
@@ -169,14 +174,19 @@ def test_host_particle_kernels_match_oracle(kind):
     assert np.all(np.diag(A) == (0.0 if kind == "coulomb" else 1.5))
 
 
-def test_matern_rejects_unsupported_nu():
+def test_matern_accepts_general_nu_and_rejects_zero():
+    """Any nu > 0 is a Matern kernel (general nu through K_nu); nu = 0
+    makes 2^(1-nu)/Gamma(nu) x^nu K_nu(x) degenerate and is refused."""
     L = S.lib()
-    data, kern = C.c_void_p(), C.c_void_p()
-    info = L.starsh_application(
-        C.byref(data), C.byref(kern), S.STARSH_int(100), C.c_char(b"d"),
-        C.c_int(S.STARSH_SPATIAL), C.c_int(S.STARSH_SPATIAL_MATERN),
-        C.c_int(S.STARSH_SPATIAL_NU), C.c_double(0.7), C.c_int(0))
-    assert info == S.STARSH_WRONG_PARAMETER
+    for nu, want in ((0.7, 0), (3.3, 0), (0.0, S.STARSH_WRONG_PARAMETER)):
+        data, kern = C.c_void_p(), C.c_void_p()
+        info = L.starsh_application(
+            C.byref(data), C.byref(kern), S.STARSH_int(100), C.c_char(b"d"),
+            C.c_int(S.STARSH_SPATIAL), C.c_int(S.STARSH_SPATIAL_MATERN),
+            C.c_int(S.STARSH_SPATIAL_NU), C.c_double(nu), C.c_int(0))
+        assert info == want, nu
+        if info == 0:
+            L.starsh_ssdata_free(data)
 
 
 def test_product_dsvfr_equals_oracle():

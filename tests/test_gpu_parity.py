@@ -54,7 +54,7 @@ def test_omega_matches_oracle():
         assert np.max(np.abs(out - ref)) <= 1e-14 * np.max(np.abs(ref)), rel
 
 
-def _gen_case(family, ndim, **kw):
+def _gen_case(family, ndim, tol=8e-16, **kw):
     rng = np.random.default_rng(7)
     pts = rng.random((ndim, 700))
     pts[:, 5] = pts[:, 300]      # a coincident pair: r == 0 off the diagonal
@@ -86,7 +86,7 @@ def _gen_case(family, ndim, **kw):
                              np.arange(c0[t], c0[t] + n[t]))
         got = out[off[t]:off[t] + m[t] * n[t]].reshape(n[t], m[t]).T
         scale = np.max(np.abs(ref))
-        assert np.max(np.abs(got - ref)) <= 8e-16 * scale, (family, t)
+        assert np.max(np.abs(got - ref)) <= tol * scale, (family, t)
 
 
 @pytest.mark.parametrize("family,ndim,kw", [
@@ -101,6 +101,21 @@ def _gen_case(family, ndim, **kw):
 def test_generated_tiles_match_oracle(family, ndim, kw):
     _say(f"case: generate {family} {ndim}d")
     _gen_case(family, ndim, **kw)
+
+
+@pytest.mark.parametrize("family,ndim,kw", [
+    ("matern", 3, dict(beta=0.1, nu=0.8, noise=0.3)),
+    ("matern", 2, dict(beta=0.05, nu=2.2)),
+    ("matern2", 3, dict(beta=0.2, nu=3.7, sigma=1.7)),
+])
+def test_generated_matern_general_nu_match_oracle(family, ndim, kw):
+    """Matern with nu outside {1/2, 3/2, 5/2}: the device evaluates the
+    reference's sigma 2^(1-nu)/Gamma(nu) x^nu K_nu(x) with its own K_nu
+    (continued fraction / trapezoid), the oracle with an independent finer
+    quadrature; both agree with scipy's kv to a few 1e-14, so the tiles
+    agree to 1e-13 of their largest entry."""
+    _say(f"case: generate {family} {ndim}d nu {kw['nu']}")
+    _gen_case(family, ndim, tol=1e-13, **kw)
 
 
 def _spectrum_matrix(m, n, sigmas, seed):
@@ -235,3 +250,35 @@ def test_compress_sqexp2d_ragged_nonsymmetric():
     _say("case: compress 2D sqexp N=3000 nb=700 symm N (ragged)")
     _compress_vs_oracle("spatial", 3000, 2, 700, 150, 16, 1e-7, symm="N",
                         seed=1, kernel="sqrexp", beta=0.1)
+
+
+@pytest.mark.parametrize("backend", ["HIP", "HIP_MIXED"])
+def test_compress_matern_general_nu(backend):
+    """3D Matern with nu = 0.8 (no closed form), N=4000 uniform points,
+    tiles 500, maxrank 200, tol 1e-7: the reference's acceptance bound and,
+    for the plain form, ranks equal to the oracle's on sampled tiles (up to
+    near-ties of the rank rule)."""
+    _say(f"case: {backend} compress 3D matern nu 0.8 N=4000 nb=500")
+    fn = S.init(backend, oversample=10)
+    p = S.TLRProblem("spatial", 4000, 3, 500, symm="S", seed=5,
+                     place=S.STARSH_PARTICLES_RAND, kernel="matern",
+                     beta=0.1, nu=0.8)
+    assert p.approximate(200, 1e-7, 0, fn) == 0
+    err = p.dfe()
+    _say(f"  error {err:.3e}, ranks {p.ranks().tolist()[:12]}")
+    assert 0 < err <= 1e-6
+    if backend == "HIP":
+        tiles, ranks = p.tiles(), p.ranks()
+        st, sz = p.cluster_ranges()
+        kd = O.kernel("matern", 3, beta=0.1, nu=0.8)
+        pick = [0, 7, 20, len(tiles) - 1]
+        ref = O.compress(kd, p.points(), st, sz, [tiles[b] for b in pick],
+                         200, 10, 1e-7, want_factors=False, want_S=True)
+        for q, b in enumerate(pick):
+            rr, rg = int(ref["rank"][q]), int(ranks[b])
+            if rr != rg:
+                s = ref["S"][q]
+                tot = np.sum(s * s)
+                assert abs(np.sum(s[rr:] ** 2) - 1e-14 * tot) <= \
+                    1e-6 * 1e-14 * tot, (tiles[b], rg, rr)
+    p.free_matrix()

@@ -70,7 +70,9 @@ def _kv(family, nu=None):
     ("matern", "matern_0.5", 0.5), ("matern", "matern_1.5", 1.5),
     ("matern", "matern_2.5", 2.5), ("matern2", "matern2_0.5", 0.5),
     ("matern2", "matern2_1.5", 1.5), ("matern2", "matern2_2.5", 2.5),
-    ("coulomb", "coulomb", 0.5), ("sin", "sin", 0.5), ("cos", "cos", 0.5)])
+    ("coulomb", "coulomb", 0.5), ("sin", "sin", 0.5), ("cos", "cos", 0.5)]
+    + [(fam, f"{fam}_{nu}", nu) for fam in ("matern", "matern2")
+       for nu in (0.3, 0.8, 1.2, 2.2, 3.7, 6.0)])
 def test_kernel_values_match_closed_forms(family, key, nu):
     g = json.load(open(os.path.join(GOLD, "kernel_golden.json")))
     kd = O.kernel(family, 3, beta=g["beta"], nu=nu, noise=g["noise"],
@@ -80,7 +82,9 @@ def test_kernel_values_match_closed_forms(family, key, nu):
         x2 = np.array([[rec["r"]], [0.0], [0.0]])
         v = O.kernel_block(kd, x1, x2, [0], [0])[0, 0]
         want = rec[key]
-        assert abs(v - want) <= 4e-14 * max(abs(want), 1e-300), \
+        # general nu: scipy's kv (AMOS) is itself accurate to a few 1e-14
+        tol = 4e-14 if nu in (0.5, 1.5, 2.5) else 2e-13
+        assert abs(v - want) <= tol * max(abs(want), 1e-300), \
             (key, rec["r"], v, want)
 
 
