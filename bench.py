@@ -6,11 +6,15 @@ on one MI355X, through starsh_blrm_approximate.
 
 A step is one full compression of that matrix.  Inputs are resident on the
 GPU during the timed device region; `value` is the device time of the
-headline form (mixed when available, otherwise plain).  The JSON line also
-carries the API wall time (incl. factor copy-back), the dominant kernel's
-roofline, and a CPU baseline: the oracle (CPU restatement of the
-reference's OpenMP RSDD) timed on a bounded sample of tiles on the host
-cores and scaled to all tiles.
+headline form (mixed when available, otherwise plain): HIP events from the
+first kernel to the last.  The JSON line also carries, per form, the wall
+time of the starsh_blrm_approximate call (`api_wall_seconds`: argument
+checks, point upload, device batches and the PCIe copy of U and V into the
+caller's malloc'd buffers) with its host-side timeline, the roofline of the
+dominant kernel (the headline form's at the top level), and a CPU baseline:
+the oracle (CPU restatement of the reference's OpenMP RSDD) on every tile
+of the workload (C2) or a spread sample (others), best of 3, one OpenMP
+thread per host core.
 
 --config selects one of the other measured workloads (same JSON layout):
   C3          3D Matern nu=1.5 beta 0.1, N=131072 uniform (srand 2), tile
@@ -36,11 +40,19 @@ sys.path[:0] = [os.path.join(ROOT, "stars-h_amd"), ROOT]
 
 PEAK_FP64 = 78.6  # TFLOP/s, MI355X dense FP64 (vector and matrix)
 
-# HBM bytes of one launch of the plain sketch kernel at C2, from the
-# FETCH_SIZE and WRITE_SIZE counters (rocprofv3 --pmc, one run per counter)
+# Per-launch counter readings of the dominant kernels at C2 (rocprofv3
+# --pmc, one step, one run per counter group; commands and raw values in
+# profiles/c2_counters.md).  HBM bytes are FETCH_SIZE and WRITE_SIZE (KiB)
 # scaled by the same counters on a 1 GiB device copy (fetch x2.00, write
-# x1.00); commands and raw values in profiles/c2_sketch_hbm_bytes.md.
-C2_SKETCH_TRAFFIC = 7.353e9
+# x1.00).
+C2_PLAIN_SKETCH_TRAFFIC = 7.332e9   # gen_gemm_kernel<2,2,256,1,false>
+C2_MIXED_PROJ_TRAFFIC = 0.660e9     # proj_i8_kernel<2,2,64,false>
+C2_MIXED_PROJ_VALU_INSTS = 1.030e9  # SQ_INSTS_VALU (wave64 instructions)
+C2_MIXED_PROJ_VALU_BUSY = 0.69      # VALUBusy at the kernel's clock
+C2_MIXED_PROJ_MFMA_UTIL = 0.18      # MfmaUtil (int8 MFMA)
+# VALU issue peak of MI355X: 256 CUs x one wave64 instruction per cycle x
+# 64 lanes x 2.4 GHz
+PEAK_VALU_LANE_INSTS = 256 * 64 * 2.4e9 / 1e12   # T lane-instructions/s
 
 WORKLOADS = {
     "C2": dict(kind="spatial", kernel="sqrexp", N=64000, ndim=2, nb=1600,
@@ -327,14 +339,48 @@ def main():
                       "bytes": nbytes,
                       "GB_per_s": round(nbytes / sec / 1e9, 1),
                       "frac_of_hbm_peak": round(nbytes / sec / 8e12, 4)}
-    dom = plain or head
-    dom_t = dom["dominant_kernel_seconds"]
-    achieved = dom["sketch_flops"] / dom_t / 1e12 if dom_t > 0 else 0.0
+    c2 = args.config == "C2"
+    rooflines = {}
+    if plain is not None:
+        # plain form: the fused sketch GEMM on FP64 MFMA; algorithmic
+        # flops 2 m n k per tile, time from HIP events around the launch
+        t = plain["sketch_kernel_seconds"]
+        a = plain["sketch_flops"] / t / 1e12 if t > 0 else 0.0
+        rooflines["plain"] = {
+            "kernel": "gen_gemm_kernel (sketch Y = A Omega, FP64 MFMA, "
+                      "tile generated in registers)",
+            "bound": "mfma", "achieved": round(a, 3), "peak": PEAK_FP64,
+            "unit": "TFLOP/s", "frac": round(a / PEAK_FP64, 4),
+            "frac_of_attainable": (round(a / probe["mfma"], 4)
+                                   if probe else None),
+            "traffic": C2_PLAIN_SKETCH_TRAFFIC if c2 else None}
+    if "mixed" in forms:
+        # mixed form: the int8 projection B^T = A^T Q dominates; it is
+        # bound by VALU issue (FP64 tile generation and digit splitting),
+        # not by MFMA or HBM: achieved = VALU lane-instructions per second
+        # (counter reading per launch / launch time from HIP events)
+        mx = forms["mixed"]
+        t = mx["proj_kernel_seconds"] / max(1, mx["dominant_kernel_launches"])
+        a = (C2_MIXED_PROJ_VALU_INSTS * 64 / t / 1e12 if t > 0 and c2
+             else None)
+        rooflines["mixed"] = {
+            "kernel": "proj_i8_kernel (projection B^T = A^T Q: FP64 tile "
+                      "generation, int8 digit products on MFMA)",
+            "bound": "valu", "achieved": round(a, 3) if a else None,
+            "peak": round(PEAK_VALU_LANE_INSTS, 2),
+            "unit": "T VALU lane-instructions/s",
+            "frac": round(a / PEAK_VALU_LANE_INSTS, 4) if a else None,
+            "valu_busy_counter": C2_MIXED_PROJ_VALU_BUSY if c2 else None,
+            "mfma_util_counter": C2_MIXED_PROJ_MFMA_UTIL if c2 else None,
+            "launch_seconds": round(t, 7),
+            "traffic": C2_MIXED_PROJ_TRAFFIC if c2 else None}
     summary = {}
     for name, r in forms.items():
         summary[name] = {k: (float(f"{v:.6g}") if isinstance(v, float)
                              else v)
                          for k, v in r.items() if k != "problems"}
+        if name in rooflines:
+            summary[name]["roofline"] = rooflines[name]
     line = {
         "metric": w["metric"],
         "value": round(head["device_seconds"], 6),
@@ -354,15 +400,12 @@ def main():
                                          forms["mixed"]["device_seconds"], 3)
                                    if "mixed" in forms and plain
                                    else None),
-        "roofline": {"kernel": "sketch GEMM Y=A*Omega (FP64 MFMA, fused "
-                               "tile generation), plain form",
-                     "bound": "mfma", "achieved": round(achieved, 3),
-                     "peak": PEAK_FP64, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP64, 4),
-                     "frac_of_attainable": (round(achieved / probe["mfma"], 4)
-                                            if probe else None),
-                     "traffic": (C2_SKETCH_TRAFFIC if args.config == "C2"
-                                 and plain is not None else None)},
+        # the headline form's dominant kernel
+        "roofline": rooflines.get(head_name),
+        "api_wall_speedup_mixed_vs_plain": (
+            round(plain["api_wall_seconds"] /
+                  forms["mixed"]["api_wall_seconds"], 3)
+            if "mixed" in forms and plain else None),
         "cpu_baseline": cpu,
         "fp64_attainable_tflops": probe,
         "tlr_matvec": matvec,

@@ -174,6 +174,15 @@ int starsh_hip_fp64_probe(double *mfma_tflops, double *valu_tflops);
  *                         converged within it is a false far-field tile
  *                         (rank -1), as for a failed dgesdd in
  *                         src/backends/sequential/dense/drsdd.c:78;
+ *   "pipeline_batches"    batches per run of tiles (0 or 1, default: all
+ *                         tiles of a run together; n > 1 splits them so
+ *                         that the copy-back of one batch overlaps the
+ *                         next, which pays only with "copy_stream" 0);
+ *   "copy_slot_mb"        slot size of the pinned copy-back ring, MiB
+ *                         (power of two, 1..64; default 32);
+ *   "copy_stream"         device-to-host transfers of the factors on the
+ *                         library stream (1, default) or on a stream of
+ *                         their own (0);
  *   "inject_failure"      test hook: the driver step with this number
  *                         (1..4) fails once with STARSH_MALLOC_ERROR
  *                         (0 clears);

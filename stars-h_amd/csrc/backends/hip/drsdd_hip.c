@@ -71,6 +71,10 @@ int starsh_hip_set_option(const char *name, double value)
         pipeline_batches = (int)value;
         return STARSH_SUCCESS;
     }
+    if(!strcmp(name, "copy_slot_mb"))
+        return starsh_hip_copy_option(0, (int)value);
+    if(!strcmp(name, "copy_stream"))
+        return starsh_hip_copy_option(1, (int)value);
     if(!strcmp(name, "inject_failure"))
         return hip_driver_inject_failure((int)value);
     if(!strcmp(name, "release_workspace"))

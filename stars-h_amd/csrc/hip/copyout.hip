@@ -10,6 +10,7 @@
 // the slot back.  Meanwhile the calling thread goes on launching the next
 // batch on the library stream.  rsdd_out_end waits for the last copies.
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 #include <algorithm>
 #include <chrono>
@@ -37,8 +38,18 @@ __global__ __launch_bounds__(256) void arena_pack_kernel(const double *src,
         dst[po+e] = src[so+e];
 }
 
-constexpr int NSLOT = 32;
-constexpr int64_t SLOT = (int64_t)1 << 20;   // doubles (8 MiB)
+constexpr int NSLOT_MAX = 512;
+constexpr int64_t RING = (int64_t)1 << 26;   // doubles of the ring (512 MiB)
+// slot size (doubles) and count; starsh_hip_set_option("copy_slot_mb")
+int64_t SLOT = (int64_t)1 << 22;             // 32 MiB
+int NSLOT = (int)(RING/SLOT);
+// Transfers on the library stream (1, default) or on a copy stream of
+// their own (0, starsh_hip_set_option("copy_stream")).  On MI355X the
+// library stream moves the C2 factors device-to-host at about 55 GB/s, the
+// second stream at about 30 GB/s, which costs more than its overlap with
+// a later batch saves; on the library stream a transfer runs after the
+// kernels queued before it.
+int use_lib_stream = 1;
 
 struct Seg
 {
@@ -51,6 +62,7 @@ struct Job
     int64_t lo = 0, hi = 0;        // arena range
     hipEvent_t ready = nullptr;     // recorded after the pack kernel
     std::vector<Seg> segs;          // increasing aoff
+    std::vector<char> touched;      // per TOUCH range: pages in (under mu)
 };
 
 // Work of the host threads: touch the destination pages of a range of a
@@ -78,26 +90,50 @@ struct Engine
     int64_t seg_cap = 0, seg_used = 0;
     // pinned ring and its events
     double *ring = nullptr;
-    hipEvent_t slot_ev[NSLOT] = {};
-    bool slot_free[NSLOT];
+    hipEvent_t slot_ev[NSLOT_MAX] = {};
+    bool slot_free[NSLOT_MAX];
     hipStream_t cs = nullptr;
     // queues
     std::mutex mu;
-    std::condition_variable cv_job, cv_piece, cv_slot;
+    std::condition_variable cv_job, cv_piece, cv_slot, cv_touch;
     std::deque<std::unique_ptr<Job>> jobs;   // all jobs of the call
     size_t next_job = 0;
     std::deque<Task> tasks;
-    int slot_left[NSLOT];            // copy tasks of a slot not yet done
+    int slot_left[NSLOT_MAX];        // copy tasks of a slot not yet done
     bool ending = false, dispatch_done = false;
     int error = STARSH_SUCCESS;
     std::thread dispatcher;
     std::vector<std::thread> workers;
     bool running = false;
-    std::chrono::steady_clock::time_point t_first, t_last;
+    // begin of the call, first queued batch, last copy done
+    std::chrono::steady_clock::time_point t_first, t_job0, t_last;
     bool any = false;
 };
 
 Engine E;
+
+// Optional timeline of the copy-back (STARSH_HIP_COPY_TRACE=file): one line
+// per event, "kind slot offset length microseconds-since-begin".
+struct TraceRec
+{
+    char kind;
+    int slot;
+    int64_t off, len;
+    double us;
+};
+std::mutex trace_mu;
+std::vector<TraceRec> trace;
+bool trace_on = false;
+
+void trace_add(char kind, int slot, int64_t off, int64_t len)
+{
+    if(!trace_on)
+        return;
+    const double us = std::chrono::duration<double, std::micro>(
+            std::chrono::steady_clock::now()-E.t_first).count();
+    std::lock_guard<std::mutex> g(trace_mu);
+    trace.push_back(TraceRec{kind, slot, off, len, us});
+}
 
 void set_error(int e)
 {
@@ -128,6 +164,9 @@ void run_task(const Task &tk)
 {
     if(tk.slot < 0)
     {
+        // the first write to a page maps and zeroes it; every copy task of
+        // the job waits until all its touch tasks are done, so these
+        // writes never land on copied data
         for_segments(*tk.job, tk.off, tk.off+tk.len,
                 [](double *d, int64_t, int64_t len) {
                     volatile char *p = (volatile char *)d;
@@ -159,11 +198,38 @@ void worker_loop()
             tk = E.tasks.front();
             E.tasks.pop_front();
         }
+        if(tk.slot >= 0)
+        {
+            // the pages of this range must be touched first (the touch
+            // tasks of a job were queued before any of its copy tasks)
+            const int64_t t0 = tk.off/TOUCH, t1 = (tk.off+tk.len-1)/TOUCH;
+            std::unique_lock<std::mutex> g(E.mu);
+            E.cv_touch.wait(g, [&] {
+                for(int64_t q = t0; q <= t1; q++)
+                    if(!tk.job->touched[q])
+                        return false;
+                return true;
+            });
+        }
+        if(tk.slot >= 0)
+            trace_add('w', tk.slot, tk.off, tk.len);
         if(tk.slot >= 0 && hipEventSynchronize(E.slot_ev[tk.slot]) !=
                 hipSuccess)
             set_error(STARSH_UNKNOWN_ERROR);
         else
+        {
+            trace_add(tk.slot >= 0 ? 'a' : 't', tk.slot, tk.off, tk.len);
             run_task(tk);
+            trace_add(tk.slot >= 0 ? 'c' : 'u', tk.slot, tk.off, tk.len);
+        }
+        if(tk.slot < 0)
+        {
+            {
+                std::lock_guard<std::mutex> g(E.mu);
+                const_cast<Job *>(tk.job)->touched[tk.off/TOUCH] = 1;
+            }
+            E.cv_touch.notify_all();
+        }
         if(tk.slot >= 0)
         {
             bool freed = false;
@@ -195,11 +261,15 @@ void dispatcher_loop()
                 break;
             j = E.jobs[E.next_job++].get();
         }
-        if(hipStreamWaitEvent(E.cs, j->ready, 0) != hipSuccess)
+        trace_add('D', -1, j->lo, j->hi-j->lo);
+        hipStream_t cs = use_lib_stream ? starsh_hip_stream() : E.cs;
+        if(!use_lib_stream && hipStreamWaitEvent(cs, j->ready, 0) !=
+                hipSuccess)
         {
             set_error(STARSH_UNKNOWN_ERROR);
             continue;
         }
+        trace_add('W', -1, j->lo, j->hi-j->lo);
         for(int64_t off = j->lo; off < j->hi; off += SLOT)
         {
             const int64_t len = std::min(SLOT, j->hi-off);
@@ -217,10 +287,11 @@ void dispatcher_loop()
                 });
                 E.slot_free[slot] = false;
             }
+            trace_add('d', slot, off, len);
             if(hipMemcpyAsync(E.ring+(int64_t)slot*SLOT, E.arena+off,
                         sizeof(double)*(size_t)len, hipMemcpyDeviceToHost,
-                        E.cs) != hipSuccess ||
-                    hipEventRecord(E.slot_ev[slot], E.cs) != hipSuccess)
+                        cs) != hipSuccess ||
+                    hipEventRecord(E.slot_ev[slot], cs) != hipSuccess)
             {
                 set_error(STARSH_UNKNOWN_ERROR);
                 std::lock_guard<std::mutex> g(E.mu);
@@ -295,9 +366,9 @@ int rsdd_out_begin(int64_t max_doubles, int64_t max_segments)
     if(E.ring == nullptr)
     {
         HIP_TRY(hipHostMalloc((void **)&E.ring,
-                    sizeof(double)*(size_t)(NSLOT*SLOT),
+                    sizeof(double)*(size_t)RING,
                     hipHostMallocDefault));
-        for(int s = 0; s < NSLOT; s++)
+        for(int s = 0; s < NSLOT_MAX; s++)
             // blocking waits: the worker threads sleep instead of spinning
             // while the calling thread launches the next batch
             HIP_TRY(hipEventCreateWithFlags(&E.slot_ev[s],
@@ -316,6 +387,8 @@ int rsdd_out_begin(int64_t max_doubles, int64_t max_segments)
     E.dispatch_done = false;
     E.error = STARSH_SUCCESS;
     E.any = false;
+    trace_on = getenv("STARSH_HIP_COPY_TRACE") != nullptr;
+    E.t_first = std::chrono::steady_clock::now();
     // workers: the cores left after the calling thread and the dispatcher
     unsigned nth = std::thread::hardware_concurrency();
     nth = nth < 4 ? 2 : (nth > 16 ? 14 : nth-2);
@@ -387,12 +460,14 @@ int rsdd_out_append(const RsddOutput *out, int ntiles, const int *m,
         if(!E.any)
         {
             E.any = true;
-            E.t_first = E.t_last = std::chrono::steady_clock::now();
+            E.t_job0 = E.t_last = std::chrono::steady_clock::now();
         }
         // the destination pages are touched while the batch's factors are
         // packed and sent
-        const Job *jp = job.get();
+        Job *jp = job.get();
         const int64_t len = jp->hi-jp->lo;
+        trace_add('j', -1, jp->lo, len);
+        jp->touched.assign((size_t)((len+TOUCH-1)/TOUCH), 0);
         for(int64_t a = 0; a < len; a += TOUCH)
             E.tasks.push_back(Task{-1, a, std::min(TOUCH, len-a), 0, jp});
         E.jobs.push_back(std::move(job));
@@ -400,6 +475,21 @@ int rsdd_out_append(const RsddOutput *out, int ntiles, const int *m,
     E.cv_piece.notify_all();
     E.cv_job.notify_one();
     return STARSH_SUCCESS;
+}
+
+static void trace_dump(void)
+{
+    const char *path = getenv("STARSH_HIP_COPY_TRACE");
+    if(path == nullptr)
+        return;
+    FILE *f = fopen(path, "w");
+    if(f == nullptr)
+        return;
+    for(const auto &r : trace)
+        fprintf(f, "%c %d %lld %lld %.1f\n", r.kind, r.slot,
+                (long long)r.off, (long long)r.len, r.us);
+    fclose(f);
+    trace.clear();
 }
 
 int rsdd_out_end(double *window_seconds, double *tail_seconds)
@@ -417,6 +507,9 @@ int rsdd_out_end(double *window_seconds, double *tail_seconds)
         w.join();
     E.workers.clear();
     const auto t1 = std::chrono::steady_clock::now();
+    trace_add('e', -1, 0, 0);
+    if(trace_on)
+        trace_dump();
     for(auto &j : E.jobs)
         if(j->ready)
             (void)hipEventDestroy(j->ready);
@@ -426,7 +519,7 @@ int rsdd_out_end(double *window_seconds, double *tail_seconds)
     {
         if(window_seconds)
             *window_seconds += std::chrono::duration<double>(E.t_last-
-                    E.t_first).count();
+                    E.t_job0).count();
         if(tail_seconds)
             *tail_seconds += std::chrono::duration<double>(t1-t0).count();
     }
@@ -447,4 +540,27 @@ void rsdd_out_release(void)
         (void)hipHostFree(E.hseg);
     E.dseg = E.hseg = nullptr;
     E.seg_cap = 0;
+}
+
+extern "C" int starsh_hip_copy_option(int which, int value)
+{
+    if(E.running)
+        return STARSH_UNKNOWN_ERROR;
+    if(which == 0)
+    {
+        // slot size in MiB: 1 .. 64, a power of two
+        if(value < 1 || value > 64 || (value & (value-1)) != 0)
+            return STARSH_WRONG_PARAMETER;
+        SLOT = (int64_t)value << 17;
+        NSLOT = (int)(RING/SLOT);
+        return STARSH_SUCCESS;
+    }
+    if(which == 1)
+    {
+        if(value != 0 && value != 1)
+            return STARSH_WRONG_PARAMETER;
+        use_lib_stream = value;
+        return STARSH_SUCCESS;
+    }
+    return STARSH_WRONG_PARAMETER;
 }

@@ -104,6 +104,9 @@ int starsh_hip_qr_panel(int width);
 int starsh_hip_proj_int8(int on);
 /* Sweep limit of the Jacobi core (1..60; 0 restores the default 60). */
 int starsh_hip_jacobi_sweeps(int sweeps);
+/* Copy-back options: which 0 = slot size in MiB (power of two, 1..64),
+ * 1 = transfers on the library stream (1) or a copy stream (0). */
+int starsh_hip_copy_option(int which, int value);
 size_t hip_driver_workspace_budget(void);
 double hip_driver_now(void);
 
