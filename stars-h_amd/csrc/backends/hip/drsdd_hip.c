@@ -407,7 +407,7 @@ static int inject_failure(int site)
  * point can be cleaned up by hip_driver_free_factors. */
 int hip_driver_finish(STARSH_blrm **matrix, STARSH_blrf *format,
         const hip_problem *hp, hip_factors *hf, int onfly,
-        starsh_hip_stats *st)
+        starsh_hip_stats *st, hip_near *nf)
 {
     STARSH_blrf *F = format;
     STARSH_problem *P = F->problem;
@@ -527,7 +527,7 @@ int hip_driver_finish(STARSH_blrm **matrix, STARSH_blrf *format,
         double *d_D = NULL;
         if(info == STARSH_SUCCESS)
         {
-            alloc_D = malloc(sizeof(double)*size_D + 1);
+            alloc_D = alloc_host(size_D);
             if(alloc_D == NULL)
                 info = STARSH_MALLOC_ERROR;
         }
@@ -538,10 +538,11 @@ int hip_driver_finish(STARSH_blrm **matrix, STARSH_blrf *format,
             info = starsh_hip_generate_tiles(&hp->kd, hp->d_xr, hp->nr,
                     hp->d_xc, hp->nc, (int)new_near, r0, c0, mm, nn, off,
                     d_D);
-        if(info == STARSH_SUCCESS)
-            info = starsh_hip_memcpy_d2h(alloc_D, d_D, sizeof(double)*size_D);
-        if(d_D != NULL)
-            starsh_hip_free(d_D);
+        /* the tiles go to alloc_D through the copy-back engine, which
+         * the caller starts once every problem is finished */
+        nf->d = d_D;
+        nf->h = alloc_D;
+        nf->n = (int64_t)size_D;
         for(bi = 0; bi < new_near && info == STARSH_SUCCESS; bi++)
         {
             int shape[2] = {mm[bi], nn[bi]};
@@ -885,6 +886,27 @@ int hip_compress(int mixed, int nprob, STARSH_blrm **matrix,
             hip_plain_tiles(&B, nt, pidx, idx, &st);
     }
     free(pidx); free(idx); free(done);
+    if(info == STARSH_SUCCESS)
+        info = hip_driver_wrap(&B);
+    if(info == STARSH_SUCCESS)
+        hip_driver_rank_stats(&B, &st);
+    /* the format rewrite and the near-field tiles, whose copy to the host
+     * joins the factors' in the copy-back engine */
+    hip_near *nf = calloc(nprob, sizeof(hip_near));
+    if(info == STARSH_SUCCESS && nf == NULL)
+        info = STARSH_MALLOC_ERROR;
+    double tf = hip_driver_now();
+    for(int q = 0; info == STARSH_SUCCESS && q < nprob; q++)
+        info = hip_driver_finish(matrix+q, format[q], B.hp+q, B.hf+q, onfly,
+                &st, nf+q);
+    for(int q = 0; out_running && info == STARSH_SUCCESS && q < nprob; q++)
+        if(nf[q].d != NULL)
+        {
+            const int64_t zero = 0;
+            info = rsdd_out_submit(nf[q].d, nf[q].n, 1, &zero, &nf[q].n,
+                    &nf[q].h);
+        }
+    st.finish_seconds = hip_driver_now()-tf;
     if(out_running)
     {
         /* the copy threads must be done with the host buffers before
@@ -895,15 +917,10 @@ int hip_compress(int mixed, int nprob, STARSH_blrm **matrix,
         if(info == STARSH_SUCCESS)
             info = e;
     }
-    if(info == STARSH_SUCCESS)
-        info = hip_driver_wrap(&B);
-    if(info == STARSH_SUCCESS)
-        hip_driver_rank_stats(&B, &st);
-    double tf = hip_driver_now();
-    for(int q = 0; info == STARSH_SUCCESS && q < nprob; q++)
-        info = hip_driver_finish(matrix+q, format[q], B.hp+q, B.hf+q, onfly,
-                &st);
-    st.finish_seconds = hip_driver_now()-tf;
+    for(int q = 0; nf && q < nprob; q++)
+        if(nf[q].d)
+            starsh_hip_free(nf[q].d);
+    free(nf);
     if(info != STARSH_SUCCESS)
         for(int q = 0; q < nprob; q++)
         {

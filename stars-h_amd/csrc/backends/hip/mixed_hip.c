@@ -291,6 +291,22 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
                     sv+(size_t)g0*NBMAX, NBMAX, st);
             g0 = g1;
         }
+        /* optional per-tile record of the rounds (STARSH_HIP_MIXED_TRACE):
+         * round, problem, tile, sketch width, rank, accepted, predicted
+         * rank, first singular values */
+        FILE *tf = getenv("STARSH_HIP_MIXED_TRACE") ?
+            fopen(getenv("STARSH_HIP_MIXED_TRACE"), round ? "a" : "w") : NULL;
+        for(STARSH_int u = 0; tf && u < npend; u++)
+        {
+            const int q = pq[u], os = B->hp[q].oversample;
+            const double pr = predicted_rank(sv+(size_t)u*NBMAX, k[u], os,
+                    B->tol[q]);
+            fprintf(tf, "%d %d %lld %d %d %d %.3g %.3e %.3e\n", round, q,
+                    (long long)pi[u], k[u], rk[u], acc[u], pr,
+                    sv[(size_t)u*NBMAX], sv[(size_t)u*NBMAX+k[u]-1]);
+        }
+        if(tf)
+            fclose(tf);
         STARSH_int nn = 0;
         for(STARSH_int u = 0; u < npend && info == STARSH_SUCCESS; u++)
             if(!acc[u])

@@ -59,7 +59,8 @@ struct Seg
 
 struct Job
 {
-    int64_t lo = 0, hi = 0;        // arena range
+    const double *src = nullptr;    // device source of the job's range
+    int64_t lo = 0, hi = 0;        // range (arena offsets, or 0..len)
     hipEvent_t ready = nullptr;     // recorded after the pack kernel
     std::vector<Seg> segs;          // increasing aoff
     std::vector<char> touched;      // per TOUCH range: pages in (under mu)
@@ -288,7 +289,7 @@ void dispatcher_loop()
                 E.slot_free[slot] = false;
             }
             trace_add('d', slot, off, len);
-            if(hipMemcpyAsync(E.ring+(int64_t)slot*SLOT, E.arena+off,
+            if(hipMemcpyAsync(E.ring+(int64_t)slot*SLOT, j->src+(off-j->lo),
                         sizeof(double)*(size_t)len, hipMemcpyDeviceToHost,
                         cs) != hipSuccess ||
                     hipEventRecord(E.slot_ev[slot], cs) != hipSuccess)
@@ -399,6 +400,8 @@ int rsdd_out_begin(int64_t max_doubles, int64_t max_segments)
     return STARSH_SUCCESS;
 }
 
+static int queue_job(std::unique_ptr<Job> job);
+
 int rsdd_out_append(const RsddOutput *out, int ntiles, const int *m,
         const int *n, const int *take, double *const *dst_u,
         double *const *dst_v)
@@ -452,9 +455,17 @@ int rsdd_out_append(const RsddOutput *out, int ntiles, const int *m,
     E.seg_used += 3*(int64_t)nseg;
     E.used = at;
     job->hi = at;
+    job->src = E.arena+job->lo;
     // arena offsets are relative to the job's range from here on
     for(auto &sg : job->segs)
         sg.aoff -= job->lo;
+    return queue_job(std::move(job));
+}
+
+// Queues a job whose ready event has been recorded: its destination pages
+// are touched while its data is sent.
+static int queue_job(std::unique_ptr<Job> job)
+{
     {
         std::lock_guard<std::mutex> g(E.mu);
         if(!E.any)
@@ -475,6 +486,29 @@ int rsdd_out_append(const RsddOutput *out, int ntiles, const int *m,
     E.cv_piece.notify_all();
     E.cv_job.notify_one();
     return STARSH_SUCCESS;
+}
+
+int rsdd_out_submit(const double *d_src, int64_t len, int nseg,
+        const int64_t *off, const int64_t *seglen, double *const *dst)
+{
+    if(!E.running)
+        return STARSH_UNKNOWN_ERROR;
+    if(len <= 0 || nseg <= 0)
+        return STARSH_SUCCESS;
+    auto job = std::make_unique<Job>();
+    job->src = d_src;
+    job->lo = 0;
+    job->hi = len;
+    for(int q = 0; q < nseg; q++)
+    {
+        if(off[q] < 0 || seglen[q] < 0 || off[q]+seglen[q] > len ||
+                (q > 0 && off[q] < off[q-1]+seglen[q-1]))
+            return STARSH_WRONG_PARAMETER;
+        job->segs.push_back(Seg{off[q], seglen[q], dst[q]});
+    }
+    HIP_TRY(hipEventCreateWithFlags(&job->ready, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(job->ready, starsh_hip_stream()));
+    return queue_job(std::move(job));
 }
 
 static void trace_dump(void)

@@ -34,9 +34,19 @@ int hip_driver_prepare(STARSH_blrf *F, int maxrank, double tol, int onfly,
 void hip_driver_release(hip_problem *hp);
 int hip_driver_alloc_factors(STARSH_blrf *F, int maxrank, hip_factors *hf);
 void hip_driver_free_factors(STARSH_blrf *F, hip_factors *hf);
+/* Near-field tiles of a problem generated on the device (d, n doubles)
+ * and their host buffer (h, owned by the returned matrix); the caller
+ * queues the copy on the copy-back engine, waits for it and frees d. */
+typedef struct hip_near
+{
+    double *d, *h;
+    int64_t n;
+} hip_near;
+
+/* Steps 3-5 (format rewrite, near-field tiles, STARSH_blrm). */
 int hip_driver_finish(STARSH_blrm **matrix, STARSH_blrf *format,
         const hip_problem *hp, hip_factors *hf, int onfly,
-        starsh_hip_stats *st);
+        starsh_hip_stats *st, hip_near *nf);
 void hip_driver_set_stats(const starsh_hip_stats *stats);
 /* Makes the driver step numbered `site` fail once with STARSH_MALLOC_ERROR
  * (1: false far-field lists, 2: near-field lists, 3: near-field device

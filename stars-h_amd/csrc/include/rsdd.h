@@ -85,6 +85,13 @@ int rsdd_out_begin(int64_t max_doubles, int64_t max_segments);
 int rsdd_out_append(const RsddOutput *out, int ntiles, const int *m,
         const int *n, const int *take, double *const *dst_u,
         double *const *dst_v);
+/*   rsdd_out_submit  queues the copy of len doubles of device memory d_src
+ *                    (filled by work already queued on the library stream,
+ *                    and kept alive until rsdd_out_end) to host: nseg
+ *                    segments, in increasing order, d_src[off[q] ..
+ *                    off[q] + seglen[q]) to dst[q]. */
+int rsdd_out_submit(const double *d_src, int64_t len, int nseg,
+        const int64_t *off, const int64_t *seglen, double *const *dst);
 int rsdd_out_end(double *window_seconds, double *tail_seconds);
 void rsdd_out_release(void);
 
