@@ -131,15 +131,43 @@ def compress(S, ps, w, fn):
     return S.approximate_batch(ps, w["maxrank"], w["tol"])
 
 
+def _set(S, name, value):
+    assert S.lib().starsh_hip_set_option(name.encode(),
+                                         ctypes.c_double(value)) == 0, name
+
+
+def _timed(S, w, fn, steps):
+    """Device seconds of `steps` compressions of fresh problems."""
+    out = []
+    for _ in range(steps):
+        ps = make_problems(S, w)
+        assert compress(S, ps, w, fn) == 0
+        out.append(S.stats().device_seconds)
+        for p in ps:
+            p.free_matrix()
+    return out
+
+
 def run_form(S, w, backend, steps, warmup, autotune):
     fn = S.init(backend, oversample=w["os"])
-    ps = make_problems(S, w)
+    mixed = backend == "HIP_MIXED"
     schedule = None
-    if backend == "HIP_MIXED" and autotune and w["kind"] != "batch":
-        # plan step (outside the timed steps): first rank bucket by trial
-        first, secs = ps[0].autotune(w["maxrank"], w["tol"])
-        schedule = {"first_bucket_width": [32, 64, 128, 256, 512][first],
-                    "candidate_device_seconds": [round(x, 6) for x in secs]}
+    if autotune:
+        # plan step (outside the timed steps): knobs by trial
+        # (starsh_hip_autotune_batch; all problems of C5 as one batch)
+        t = S.autotune_batch(make_problems(S, w), w["maxrank"], w["tol"],
+                             form=1 if mixed else 0)
+        schedule = {k: (round(v, 6) if isinstance(v, float) else
+                        [round(x, 6) for x in v] if isinstance(v, list)
+                        else v) for k, v in t.items()}
+        if mixed:
+            schedule["first_bucket_width"] = [32, 64, 128, 256, 512][
+                t["first_bucket"]]
+    if mixed:
+        # the timed steps do not replay the per-tile outcomes recorded on
+        # this matrix by the plan step (reported apart below)
+        _set(S, "tile_schedule", 0)
+    ps = make_problems(S, w)
     dev, wall, dom, proj = [], [], [], []
     parts = ("prepare", "batch", "d2h", "scatter", "finish")
     tl = {k: [] for k in parts}
@@ -191,6 +219,8 @@ def run_form(S, w, backend, steps, warmup, autotune):
         "tiles": int(st.ntiles),
         "tiles_per_second": st.ntiles / t if t > 0 else 0.0,
         "false_far": int(st.nfalse_far),
+        "early_dense": int(st.early_dense),
+        "tile_rounds": int(st.tile_rounds),
         "rank_min": int(min(ranks)) if ranks else 0,
         "rank_max": int(max(ranks)) if ranks else 0,
         "rank_avg": float(sum(ranks) / len(ranks)) if ranks else 0.0,
@@ -200,6 +230,18 @@ def run_form(S, w, backend, steps, warmup, autotune):
     }
     if schedule is not None:
         res["schedule"] = schedule
+    if mixed:
+        _set(S, "tile_schedule", 1)
+        if schedule is not None and schedule["tile_schedule"]:
+            # recompression of the same matrix replaying the per-tile
+            # schedule of the plan step (each tile starts in the bucket it
+            # finished in; tiles found false far-field are stored densely
+            # at once): the case of a repeated compression of one problem
+            _timed(S, w, fn, 1)
+            rs = _timed(S, w, fn, steps)
+            res["tile_schedule_replay_device_seconds"] = sum(rs) / len(rs)
+            res["step_tile_schedule_replay_device_seconds"] = [
+                round(x, 6) for x in rs]
     return res
 
 

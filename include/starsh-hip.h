@@ -96,6 +96,9 @@ typedef struct starsh_hip_stats
     double finish_seconds;   /* false far-field rewrite, near tiles, blrm */
     double proj_kernel_seconds; /* summed duration of the projection
                                    kernels B^T = A^T Q (HIP events) */
+    long long early_dense;   /* mixed form: tiles declared false far-field
+                                from their extrapolated rank */
+    long long tile_rounds;   /* mixed form: tiles summed over rounds */
 } starsh_hip_stats;
 
 int starsh_hip_get_stats(starsh_hip_stats *stats);
@@ -174,6 +177,15 @@ int starsh_hip_fp64_probe(double *mfma_tflops, double *valu_tflops);
  *                         converged within it is a false far-field tile
  *                         (rank -1), as for a failed dgesdd in
  *                         src/backends/sequential/dense/drsdd.c:78;
+ *   "mixed_early_dense"   the mixed form declares a tile false far-field
+ *                         when its rank, extrapolated from the decay of
+ *                         its sketch's singular values, exceeds this
+ *                         factor times maxrank (default 1.5; 0 disables,
+ *                         values in (0, 1) are refused);
+ *   "tile_schedule"       per-tile schedules recorded by the autotuner
+ *                         used by the mixed form (1, default) or not (0);
+ *   "schedule_clear"      (value ignored) empties the autotuner's
+ *                         in-process schedule table;
  *   "pipeline_batches"    batches per run of tiles (0 or 1, default: all
  *                         tiles of a run together; n > 1 splits them so
  *                         that the copy-back of one batch overlaps the
@@ -205,6 +217,53 @@ int starsh_hip_set_option(const char *name, double value);
  * it finds false far-field tiles), and *format is left as passed. */
 int starsh_hip_autotune(struct starsh_blrf *format, int maxrank, double tol,
         int onfly, int *first_out, double *seconds_out);
+
+/* Result of starsh_hip_autotune_batch. */
+typedef struct starsh_hip_tuning
+{
+    int form;               /* 0 plain FP64, 1 mixed */
+    int first_bucket;       /* mixed: chosen first bucket (0..4), else -1 */
+    int tile_schedule;      /* mixed: per-tile schedule kept (1) or not */
+    int qr_panel;           /* chosen Gram-Schmidt panel width (0: by batch,
+                             * 4, 8) */
+    int sketch_bn, sketch_rw;   /* plain: sketch GEMM column block and
+                                 * 16-row groups per wave (0: by size) */
+    int proj_bn, proj_rw;       /* plain: projection GEMM, the same */
+    int ntrials;            /* compressions run, warm-up included */
+    double first_seconds[5];    /* mixed: device time per first bucket
+                                 * tried, -1 if not tried */
+    double schedule_seconds;    /* mixed: device time with the per-tile
+                                 * schedule, -1 if none was recorded */
+    double untuned_seconds;     /* device time with the built-in choices */
+    double best_seconds;        /* device time with the chosen knobs */
+} starsh_hip_tuning;
+
+/* Autotuner of both HIP forms for one problem or a batch (no reference
+ * counterpart); form 0 tunes starsh_blrm__drsdd_hip, 1 the mixed form.
+ * The nprob problems are compressed together (as by
+ * starsh_blrm_approximate_batch) once per trial, each trial timed on the
+ * device and run on the formats as passed (restored after each trial):
+ *   plain: sketch GEMM (column block 256 or 128, one or two 16-row groups
+ *          per wave), then projection GEMM (128 or 64, the same), then the
+ *          Gram-Schmidt panel width (4, 8), each kept if faster;
+ *   mixed: the first bucket as starsh_hip_autotune; then one compression
+ *          at that bucket recording every far-field tile's outcome (the
+ *          bucket it finished in, or false far-field) and one with that
+ *          per-tile schedule (every tile starts in its bucket, false
+ *          far-field tiles are stored densely at once), kept if faster;
+ *          then the panel width.
+ * The chosen knobs go to the schedule table (and to STARSH_HIP_SCHEDULE's
+ * file) under each problem's signature; later compressions of that form
+ * use them, options set by starsh_hip_set_option taking precedence.  A
+ * per-tile schedule also applies to a problem with the same point
+ * coordinates (hashed), kernel family, tile size, maxrank, oversample,
+ * tolerance and number of far-field tiles but other kernel parameters (a
+ * parameter sweep over one point set): it only moves work between buckets
+ * and stores tiles densely, so it can cost time, never accuracy.  Option
+ * "tile_schedule" 0 turns per-tile schedules off. */
+int starsh_hip_autotune_batch(struct starsh_blrf **format, int nprob,
+        const int *maxrank, const double *tol, int onfly, int form,
+        starsh_hip_tuning *out);
 
 /* ---- Thin C ABI over the HIP kernels.  Plain pointers and sizes; device
  * pointers are marked d_.  Every function returns STARSH_SUCCESS or a

@@ -64,6 +64,17 @@ int starsh_hip_set_option(const char *name, double value)
         return starsh_hip_proj_int8((int)value);
     if(!strcmp(name, "jacobi_max_sweeps"))
         return starsh_hip_jacobi_sweeps((int)value);
+    if(!strcmp(name, "mixed_trunc"))
+        return hip_mixed_set_eta(value);
+    if(!strcmp(name, "mixed_early_dense"))
+        return hip_mixed_set_early_dense(value);
+    if(!strcmp(name, "tile_schedule"))
+        return hip_mixed_set_tile_schedule((int)value);
+    if(!strcmp(name, "schedule_clear"))
+    {
+        hip_sched_clear();
+        return STARSH_SUCCESS;
+    }
     if(!strcmp(name, "pipeline_batches"))
     {
         if(value < 0 || value > 64)
@@ -609,7 +620,7 @@ static size_t tile_bytes(int m, int n, int k, int maxrank)
 }
 
 int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
-        const STARSH_int *idx, const int *k, int int8, double tau,
+        const STARSH_int *idx, const int *k, int int8, double tau, double eta,
         int *rank_out, int *flags_out, int *accepted, hip_accept_fn accept,
         double *S_out, int sstride, starsh_hip_stats *st)
 {
@@ -693,6 +704,7 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
         in.c0 = c0+b0;
         in.sketch_int8 = int8;
         in.tau = tau;
+        in.eta = eta;
         if(S_out)
         {
             in.S_out = S_out;
@@ -802,7 +814,7 @@ int hip_plain_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
     for(STARSH_int t = 0; info == STARSH_SUCCESS && t < ntiles; t++)
         k[t] = B->maxrank[pidx[t]]+B->hp[pidx[t]].oversample;
     if(info == STARSH_SUCCESS)
-        info = hip_driver_run(B, ntiles, pidx, idx, k, 0, 0.0, rk, fl, acc,
+        info = hip_driver_run(B, ntiles, pidx, idx, k, 0, 0.0, 0.0, rk, fl, acc,
                 NULL, NULL, 0, st);
     free(k); free(rk); free(fl); free(acc);
     return info;
@@ -856,6 +868,9 @@ int hip_compress(int mixed, int nprob, STARSH_blrm **matrix,
         info = rsdd_out_begin(nd, ns);
         out_running = info == STARSH_SUCCESS;
     }
+    /* knobs recorded by the autotuner for this problem, if any */
+    if(info == STARSH_SUCCESS)
+        hip_sched_apply(mixed, &B);
     st.prepare_seconds = hip_driver_now()-t_start;
     /* all far-field tiles, grouped by kernel family and dimension */
     STARSH_int ntot = 0;

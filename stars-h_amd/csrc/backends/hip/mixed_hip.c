@@ -17,128 +17,48 @@
  * All pending tiles of a bucket go through the pipeline as one batch. */
 #include <math.h>
 #include "common.h"
+#include "kernels_host.h"
 #include "rsdd.h"
 #include "hip_driver.h"
 
 static const int bucket_widths[] = {32, 64, 128, 256, 512};
 #define NBUCKETS ((int)(sizeof(bucket_widths)/sizeof(bucket_widths[0])))
 
-/* First bucket forced by starsh_hip_set_option("mixed_first_bucket"), or -1
- * for the schedule table, then DEFAULT_FIRST (width 64) when the problem has
- * no entry: on every workload measured (2D sqexp and exp, 3D Matern,
- * Coulomb and sin(kr)/r) width 64 beat width 32 as the first bucket. */
-#define DEFAULT_FIRST 1
-static int forced_bucket = -1;
-/* Tiles still pending after the first bucket of the last call. */
-static long long last_pending_after_first = 0;
+/* Early false far-field decision: a tile that has not finished and whose
+ * rank, extrapolated from the singular values of its sketch, exceeds
+ * early_dense times maxrank is declared a false far-field tile (a dense
+ * near-field tile, exact) without the full-width round that the reference
+ * rule would take to reach the same verdict.  0 disables it.  On the 3D
+ * Matern workload (N=131072, tile 2048, maxrank 512, first width 256) the
+ * 70 false far-field tiles extrapolate to ranks 470-850 and the tiles that
+ * finish with ranks up to 512 to at most 463; 1.5 x maxrank = 768 separates
+ * them with a wide margin; on the sin(kr)/r workload (tile 1536, maxrank
+ * 384) every false far-field tile above 576 is caught and no other tile.
+ * Tiles no wider than maxrank are never declared this way. */
+static double early_dense = 1.5;
 
-int hip_mixed_set_first_bucket(int b)
+/* Truncation of B's rows in the mixed form: the rows dropped before the
+ * core SVD carry at most (mixed_eta tol)^2 |B|_F^2, counted in full in the
+ * rank rule's tail, so the rank can only grow and the error bound holds.
+ * The plain form keeps 1e-5 (the reference's ranks up to near-ties). */
+static double mixed_eta = 1e-5;
+
+int hip_mixed_set_eta(double e)
 {
-    if(b < -1 || b >= NBUCKETS)
+    if(!(e > 0.0 && e <= 0.5))
         return STARSH_WRONG_PARAMETER;
-    forced_bucket = b;
+    mixed_eta = e;
     return STARSH_SUCCESS;
 }
 
-/* Schedule table of the autotuner: the first bucket that gave the shortest
- * compression time for a problem signature (kernel, its parameters, point
- * count, largest cluster, maxrank, oversample, tolerance).  Entries come
- * from starsh_hip_autotune() and from the text file named by the
- * environment variable STARSH_HIP_SCHEDULE, one entry per line:
- *   family ndim beta nu noise sigma wave_k diag n nb maxrank os tol first */
-typedef struct sched_entry
+int hip_mixed_set_early_dense(double f)
 {
-    starsh_hip_kernel_desc kd;
-    long long n;
-    int nb, maxrank, os;
-    double tol;
-    int first;
-} sched_entry;
-
-#define SCHED_MAX 256
-static sched_entry sched[SCHED_MAX];
-static int nsched = 0, sched_loaded = 0;
-
-static void sched_key(const hip_problem *hp, const STARSH_blrf *F,
-        int maxrank, double tol, sched_entry *e)
-{
-    memset(e, 0, sizeof(*e));
-    e->kd = hp->kd;
-    e->n = hp->nr;
-    int nb = 0;
-    for(STARSH_int i = 0; i < F->row_cluster->nblocks; i++)
-        if(F->row_cluster->size[i] > nb)
-            nb = (int)F->row_cluster->size[i];
-    e->nb = nb;
-    e->maxrank = maxrank;
-    e->os = hp->oversample;
-    e->tol = tol;
-    e->first = -1;
+    if(!(f >= 0.0) || (f > 0.0 && f < 1.0))
+        return STARSH_WRONG_PARAMETER;
+    early_dense = f;
+    return STARSH_SUCCESS;
 }
 
-static int sched_same(const sched_entry *a, const sched_entry *b)
-{
-    const starsh_hip_kernel_desc *x = &a->kd, *y = &b->kd;
-    return x->family == y->family && x->ndim == y->ndim &&
-        x->beta == y->beta && x->nu == y->nu && x->noise == y->noise &&
-        x->sigma == y->sigma && x->wave_k == y->wave_k &&
-        x->diag == y->diag && a->n == b->n && a->nb == b->nb &&
-        a->maxrank == b->maxrank && a->os == b->os && a->tol == b->tol;
-}
-
-static void sched_put(const sched_entry *e)
-{
-    for(int i = 0; i < nsched; i++)
-        if(sched_same(&sched[i], e))
-        {
-            sched[i].first = e->first;
-            return;
-        }
-    if(nsched < SCHED_MAX)
-        sched[nsched++] = *e;
-}
-
-static void sched_load(void)
-{
-    if(sched_loaded)
-        return;
-    sched_loaded = 1;
-    const char *path = getenv("STARSH_HIP_SCHEDULE");
-    FILE *f = path ? fopen(path, "r") : NULL;
-    if(f == NULL)
-        return;
-    sched_entry e;
-    memset(&e, 0, sizeof(e));
-    while(fscanf(f, "%d %d %lg %lg %lg %lg %lg %lg %lld %d %d %d %lg %d",
-                &e.kd.family, &e.kd.ndim, &e.kd.beta, &e.kd.nu, &e.kd.noise,
-                &e.kd.sigma, &e.kd.wave_k, &e.kd.diag, &e.n, &e.nb,
-                &e.maxrank, &e.os, &e.tol, &e.first) == 14)
-        if(e.first >= 0 && e.first < NBUCKETS)
-            sched_put(&e);
-    fclose(f);
-}
-
-static void sched_save(const sched_entry *e)
-{
-    const char *path = getenv("STARSH_HIP_SCHEDULE");
-    FILE *f = path ? fopen(path, "a") : NULL;
-    if(f == NULL)
-        return;
-    fprintf(f, "%d %d %.17g %.17g %.17g %.17g %.17g %.17g %lld %d %d %d "
-            "%.17g %d\n", e->kd.family, e->kd.ndim, e->kd.beta, e->kd.nu,
-            e->kd.noise, e->kd.sigma, e->kd.wave_k, e->kd.diag, e->n, e->nb,
-            e->maxrank, e->os, e->tol, e->first);
-    fclose(f);
-}
-
-static int sched_first(const sched_entry *key)
-{
-    sched_load();
-    for(int i = 0; i < nsched; i++)
-        if(sched_same(&sched[i], key))
-            return sched[i].first;
-    return -1;
-}
 
 /* A randomized range finder with k = r + p samples leaves an expected
  * error factor sqrt(1 + r/(p-1)) over the best rank-r approximation; the
@@ -232,27 +152,33 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
     const int int8 = tmin >= 3e-8;
     if(!int8)
         tau = 1e-13;
+    /* first bucket and per-tile schedule of each problem (autotuner) */
+    const signed char **tsched = calloc(B->nprob, sizeof(*tsched));
+    if(tsched == NULL)
+        info = STARSH_MALLOC_ERROR;
     for(int q = 0; info == STARSH_SUCCESS && q < B->nprob; q++)
+        hip_sched_plan(B, q, first+q, tsched+q);
+    STARSH_int npend = 0;
+    for(STARSH_int t = 0; info == STARSH_SUCCESS && t < ntiles; t++)
     {
-        first[q] = forced_bucket;
-        if(first[q] < 0)
+        const int q = pidx[t];
+        const int b = tsched[q] ? tsched[q][idx[t]] : first[q];
+        if(b == -2)
         {
-            sched_entry key;
-            sched_key(B->hp+q, B->F[q], B->maxrank[q], B->tol[q], &key);
-            first[q] = sched_first(&key);
-            if(first[q] < 0)
-                first[q] = DEFAULT_FIRST;
+            /* a false far-field tile in the tuning run: dense at once */
+            B->hf[q].far_rank[idx[t]] = -1;
+            st->early_dense++;
+            hip_sched_record(q, idx[t], -2);
+            continue;
         }
+        pend[npend] = t;
+        bucket[t] = b;
+        npend++;
     }
-    STARSH_int npend = info == STARSH_SUCCESS ? ntiles : 0;
-    for(STARSH_int t = 0; t < npend; t++)
-    {
-        pend[t] = t;
-        bucket[t] = first[pidx[t]];
-    }
-    last_pending_after_first = npend;
+    hip_sched_pending_after_first(npend);
     for(int round = 0; npend > 0 && info == STARSH_SUCCESS; round++)
     {
+        st->tile_rounds += npend;
         /* pending tiles grouped by bucket (stable), one batch per group so
          * that every batch has one sketch width class */
         STARSH_int cnt[NBUCKETS+1];
@@ -287,6 +213,7 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
             while(g1 < npend && bucket[pend[g1]] == bucket[pend[g0]])
                 g1++;
             info = hip_driver_run(B, g1-g0, pq+g0, pi+g0, k+g0, int8, tau,
+                    mixed_eta,
                     rk+g0, fl+g0, acc+g0, accept_mixed,
                     sv+(size_t)g0*NBMAX, NBMAX, st);
             g0 = g1;
@@ -307,6 +234,10 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
         }
         if(tf)
             fclose(tf);
+        for(STARSH_int u = 0; u < npend; u++)
+            if(acc[u])
+                hip_sched_record(pq[u], pi[u], rk[u] < 0 ? -2 :
+                        bucket[pend[u]]);
         STARSH_int nn = 0;
         for(STARSH_int u = 0; u < npend && info == STARSH_SUCCESS; u++)
             if(!acc[u])
@@ -318,6 +249,24 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
                 const int q = pq[u], os = B->hp[q].oversample;
                 double r = predicted_rank(sv+(size_t)u*NBMAX, k[u], os,
                         B->tol[q]);
+                /* 1e9 means "decay too slow to extrapolate": no verdict;
+                 * a tile with min(m, n) <= maxrank cannot exceed maxrank
+                 * (the ragged 1536 x 160 tiles of the sin(kr)/r workload
+                 * extrapolate far beyond their 160 columns) */
+                const STARSH_blrf *Fq = B->F[q];
+                const STARSH_int mt = Fq->row_cluster->size[
+                    Fq->block_far[2*pi[u]]];
+                const STARSH_int nt = Fq->col_cluster->size[
+                    Fq->block_far[2*pi[u]+1]];
+                const STARSH_int mnt = mt < nt ? mt : nt;
+                if(early_dense > 0.0 && r < 1e9 && mnt > B->maxrank[q] &&
+                        r > early_dense*B->maxrank[q])
+                {
+                    B->hf[q].far_rank[pi[u]] = -1;
+                    st->early_dense++;
+                    hip_sched_record(q, pi[u], -2);
+                    continue;
+                }
                 if(r < rk[u])
                     r = rk[u];
                 const double want = r+(r > os ? r : os);
@@ -332,146 +281,9 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
         next = tmp;
         npend = nn;
         if(round == 0)
-            last_pending_after_first = nn;
+            hip_sched_pending_after_first(nn);
     }
     free(pend); free(next); free(bucket); free(pq); free(pi); free(k);
-    free(rk); free(fl); free(acc); free(first); free(sv);
+    free(rk); free(fl); free(acc); free(first); free(sv); free(tsched);
     return info;
-}
-
-/* Runs the mixed form once per candidate first bucket (after one untimed
- * run), keeps the fastest by device time, records it in the schedule table
- * (and in the STARSH_HIP_SCHEDULE file) and returns it in *first_out.  The
- * candidates stop at the first bucket that finishes every tile. */
-/* The tile lists of a format, kept so that every trial of the autotuner
- * starts from the format the caller passed (a compression rewrites it when
- * it finds false far-field tiles). */
-typedef struct
-{
-    STARSH_int nfar, nnear;
-    STARSH_int *far, *near;
-} format_lists;
-
-static int lists_save(const STARSH_blrf *F, format_lists *L)
-{
-    L->nfar = F->nblocks_far;
-    L->nnear = F->nblocks_near;
-    L->far = malloc(sizeof(STARSH_int)*2*(L->nfar > 0 ? L->nfar : 1));
-    L->near = malloc(sizeof(STARSH_int)*2*(L->nnear > 0 ? L->nnear : 1));
-    if(!L->far || !L->near)
-        return STARSH_MALLOC_ERROR;
-    if(L->nfar > 0)
-        memcpy(L->far, F->block_far, sizeof(STARSH_int)*2*L->nfar);
-    if(L->nnear > 0)
-        memcpy(L->near, F->block_near, sizeof(STARSH_int)*2*L->nnear);
-    return STARSH_SUCCESS;
-}
-
-/* Rebuilds *F from the saved lists if a trial changed them. */
-static int lists_restore(STARSH_blrf *F, const format_lists *L)
-{
-    if(F->nblocks_far == L->nfar && F->nblocks_near == L->nnear)
-        return STARSH_SUCCESS;
-    STARSH_int *far = malloc(sizeof(STARSH_int)*2*(L->nfar > 0 ? L->nfar :
-                1));
-    STARSH_int *near = malloc(sizeof(STARSH_int)*2*(L->nnear > 0 ?
-                L->nnear : 1));
-    if(!far || !near)
-    {
-        free(far);
-        free(near);
-        return STARSH_MALLOC_ERROR;
-    }
-    memcpy(far, L->far, sizeof(STARSH_int)*2*L->nfar);
-    memcpy(near, L->near, sizeof(STARSH_int)*2*L->nnear);
-    STARSH_blrf *F2;
-    int info = starsh_blrf_new_from_coo(&F2, F->problem, F->symm,
-            F->row_cluster, F->col_cluster, L->nfar, far, L->nnear, near,
-            F->type);
-    if(info != STARSH_SUCCESS)
-        return info;
-    if(L->nfar == 0)
-        free(far);
-    if(L->nnear == 0)
-        free(near);
-    STARSH_blrf tmp = *F;
-    *F = *F2;
-    *F2 = tmp;
-    starsh_blrf_free(F2);
-    return STARSH_SUCCESS;
-}
-
-int starsh_hip_autotune(STARSH_blrf *format, int maxrank, double tol,
-        int onfly, int *first_out, double *seconds_out)
-{
-    if(format == NULL)
-        return STARSH_WRONG_PARAMETER;
-    format_lists lists = {0, 0, NULL, NULL};
-    int info = lists_save(format, &lists);
-    if(info != STARSH_SUCCESS)
-    {
-        free(lists.far);
-        free(lists.near);
-        return info;
-    }
-    const int saved = forced_bucket;
-    STARSH_blrm *M = NULL;
-    forced_bucket = 0;
-    info = starsh_blrm__drsdd_hip_mixed(&M, format, maxrank, tol, onfly);
-    if(info == STARSH_SUCCESS)
-    {
-        starsh_blrm_free(M);
-        info = lists_restore(format, &lists);
-    }
-    if(info != STARSH_SUCCESS)
-    {
-        forced_bucket = saved;
-        free(lists.far);
-        free(lists.near);
-        return info;
-    }
-    int best = 0;
-    double best_t = 1e300;
-    for(int b = 0; b < NBUCKETS; b++)
-    {
-        forced_bucket = b;
-        M = NULL;
-        info = starsh_blrm__drsdd_hip_mixed(&M, format, maxrank, tol, onfly);
-        if(info != STARSH_SUCCESS)
-            break;
-        starsh_hip_stats st;
-        starsh_hip_get_stats(&st);
-        starsh_blrm_free(M);
-        info = lists_restore(format, &lists);
-        if(info != STARSH_SUCCESS)
-            break;
-        if(seconds_out)
-            seconds_out[b] = st.device_seconds;
-        if(st.device_seconds < best_t)
-        {
-            best_t = st.device_seconds;
-            best = b;
-        }
-        if(last_pending_after_first == 0)
-            break;
-    }
-    forced_bucket = saved;
-    free(lists.far);
-    free(lists.near);
-    if(info != STARSH_SUCCESS)
-        return info;
-    hip_problem hp;
-    info = hip_driver_prepare(format, maxrank, tol, onfly, &hp);
-    if(info != STARSH_SUCCESS)
-        return info;
-    sched_entry e;
-    sched_key(&hp, format, maxrank, tol, &e);
-    hip_driver_release(&hp);
-    e.first = best;
-    sched_load();
-    sched_put(&e);
-    sched_save(&e);
-    if(first_out)
-        *first_out = best;
-    return STARSH_SUCCESS;
 }

@@ -19,6 +19,10 @@ constexpr int BK = 16;
 // choice by size below; set by starsh_hip_set_option.
 int gen_gemm_bn = 0;
 int gen_gemm_rw = 0;
+// The same, chosen by the autotuner for the current call, per operation
+// ([0] sketch, [1] projection; 0 = by size); the options above win.
+int gen_gemm_tuned_bn[2] = {0, 0};
+int gen_gemm_tuned_rw[2] = {0, 0};
 
 // PROJ=false: rows are row points, K runs over column points.
 // PROJ=true : rows are column points, K runs over row points (A^T).
@@ -267,6 +271,10 @@ int launch_nd(const TileDesc *d_tiles, int ntiles, int mmax, int nmax,
         bn = nmax > 128 ? 256 : (nmax > 64 ? 128 : 64);
         rw = 1;
     }
+    if(gen_gemm_tuned_bn[PROJ] > 0)
+        bn = gen_gemm_tuned_bn[PROJ];
+    if(gen_gemm_tuned_rw[PROJ] > 0)
+        rw = gen_gemm_tuned_rw[PROJ];
     if(gen_gemm_bn > 0)
         bn = gen_gemm_bn;
     if(gen_gemm_rw > 0)
@@ -304,6 +312,16 @@ extern "C" int starsh_hip_gen_gemm_rw(int rw)
     if(rw != 0 && rw != 1 && rw != 2)
         return STARSH_WRONG_PARAMETER;
     gen_gemm_rw = rw;
+    return STARSH_SUCCESS;
+}
+
+extern "C" int hip_set_tuned_gemm(int proj, int bn, int rw)
+{
+    if((proj != 0 && proj != 1) || (bn != 0 && bn != 64 && bn != 128 &&
+                bn != 256) || (rw != 0 && rw != 1 && rw != 2))
+        return STARSH_WRONG_PARAMETER;
+    gen_gemm_tuned_bn[proj] = bn;
+    gen_gemm_tuned_rw[proj] = rw;
     return STARSH_SUCCESS;
 }
 

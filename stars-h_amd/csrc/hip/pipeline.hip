@@ -21,7 +21,8 @@ namespace {
 // norm before projection, and to 1 for the second pass).
 constexpr double TAU_FIRST = 1e-13;
 constexpr double TAU_SECOND = 1e-8;
-// Truncation of B's rows: discarded tail at most (ETA*tol)^2 |B|_F^2.
+// Truncation of B's rows: discarded tail at most (ETA*tol)^2 |B|_F^2
+// (RsddInput.eta overrides it).
 constexpr double ETA = 1e-5;
 
 struct DeviceBuf
@@ -122,6 +123,9 @@ ActiveProbe probe;
 // 10.1 ms against 11.7 ms with 8 columns); larger ones take 8 columns (C3,
 // 2080 tiles: 1.61 s against 1.72 s in the plain form).
 int qr_panel_width = 0;
+// The width chosen by the autotuner for the current call (0 = by batch);
+// the option above wins.
+int qr_panel_tuned = 0;
 constexpr int PANEL8_TILES = 1536;
 
 // Projection of the mixed form on int8 digit planes (1, default) or FP64
@@ -132,8 +136,8 @@ int bcgs2(TileDesc *d_tiles, int ntiles, int which, double *ws, int kmax,
         int width_max, int rows_max, int *d_keep, double tau_first,
         int *d_active, hipStream_t s)
 {
-    const bool wide = qr_panel_width == 8 ||
-        (qr_panel_width == 0 && ntiles > PANEL8_TILES);
+    const int pw = qr_panel_width ? qr_panel_width : qr_panel_tuned;
+    const bool wide = pw == 8 || (pw == 0 && ntiles > PANEL8_TILES);
     const int pb = rows_max <= 2048 && wide ? 8 : 4;
     TRY(probe.init());
     TRY(pipeline_count_active(d_tiles, ntiles, which, d_active, s));
@@ -462,7 +466,7 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     // 4. truncation of B's rows
     TRY(pipeline_colnorm(d_tiles, nt, 2, d_ws, (double *)norms.p, kmax, s));
     TRY(pipeline_truncate(d_tiles, nt, d_ws, (double *)norms.p, kmax,
-                ETA, nmax, s));
+                in->eta > 0 ? in->eta : ETA, nmax, s));
     HIP_TRY(hipMemcpyAsync(h.data(), d_tiles, sizeof(TileDesc)*nt,
                 hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -534,6 +538,14 @@ extern "C" int starsh_hip_proj_int8(int on)
     if(on != 0 && on != 1)
         return STARSH_WRONG_PARAMETER;
     proj_int8 = on;
+    return STARSH_SUCCESS;
+}
+
+extern "C" int hip_set_tuned_panel(int width)
+{
+    if(width != 0 && width != 4 && width != 8)
+        return STARSH_WRONG_PARAMETER;
+    qr_panel_tuned = width;
     return STARSH_SUCCESS;
 }
 

@@ -77,7 +77,7 @@ typedef int (*hip_accept_fn)(int rank, int k, int kfull, int flags,
  * not NULL the k[t] singular values of tile t (those of B, then the norm
  * of the truncated rows, then zeros) go to S_out + t*sstride. */
 int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
-        const STARSH_int *idx, const int *k, int int8, double tau,
+        const STARSH_int *idx, const int *k, int int8, double tau, double eta,
         int *rank_out, int *flags_out, int *accepted, hip_accept_fn accept,
         double *S_out, int sstride, starsh_hip_stats *st);
 void hip_driver_rank_stats(const hip_batch *B, starsh_hip_stats *st);
@@ -103,6 +103,11 @@ int hip_compress(int mixed, int nprob, STARSH_blrm **matrix,
 
 /* First rank bucket (index into 32, 64, 128, 256, 512) of the mixed form. */
 int hip_mixed_set_first_bucket(int b);
+/* Early false far-field factor of the mixed form (0: off). */
+int hip_mixed_set_early_dense(double f);
+/* Row truncation factor of the mixed form (B's rows dropped before the
+ * core carry at most (eta tol)^2 |B|^2). */
+int hip_mixed_set_eta(double e);
 
 /* Column block of the generated-operand GEMMs (0 = by size, 64/128/256). */
 int starsh_hip_gen_gemm_bn(int bn);
@@ -110,6 +115,27 @@ int starsh_hip_gen_gemm_bn(int bn);
 int starsh_hip_gen_gemm_rw(int rw);
 /* Panel width of the block Gram-Schmidt (8 or 4). */
 int starsh_hip_qr_panel(int width);
+/* Values chosen by the autotuner for the current call (0 = by size); the
+ * options above take precedence.  proj: 0 sketch GEMM, 1 projection. */
+int hip_set_tuned_gemm(int proj, int bn, int rw);
+int hip_set_tuned_panel(int width);
+/* Applies the autotuner's knobs recorded for problem 0 of a batch (or the
+ * trial knobs while tuning) before a compression; mixed: 0 plain, 1 mixed
+ * form. */
+void hip_sched_apply(int mixed, const hip_batch *B);
+/* Per-tile schedules of the mixed form on (1, default) or off (0). */
+int hip_mixed_set_tile_schedule(int on);
+/* First bucket and per-tile schedule (or NULL) of problem q of a mixed
+ * compression. */
+void hip_sched_plan(const hip_batch *B, int q, int *first,
+        const signed char **tiles);
+/* Outcome of far-field tile `tile` of problem q (final bucket, or -2 for a
+ * false far-field tile), kept while the autotuner records. */
+void hip_sched_record(int q, STARSH_int tile, int outcome);
+/* Empties the in-process schedule table (the file is not read again). */
+void hip_sched_clear(void);
+/* Tiles pending after the first bucket of the last mixed call. */
+void hip_sched_pending_after_first(long long n);
 /* Projection of the mixed form on int8 digit planes (1) or FP64 MFMA (0). */
 int starsh_hip_proj_int8(int on);
 /* Sweep limit of the Jacobi core (1..60; 0 restores the default 60). */

@@ -43,6 +43,9 @@ typedef struct rsdd_input
     const int *k;                /* optional per-tile sketch width (default
                                     min(maxrank+oversample, m, n)) */
     int sketch_int8;             /* 1: int8-digit sketch (mixed form) */
+    double eta;                  /* truncation of B's rows: discarded
+                                    mass at most (eta tol)^2 |B|_F^2
+                                    (0: 1e-5) */
     double tau;                  /* deflation threshold of the sketch
                                     columns (0: 1e-13) */
     double *S_out;               /* optional: k singular values per tile */

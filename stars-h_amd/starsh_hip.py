@@ -127,7 +127,9 @@ class Stats(C.Structure):
                 ("d2h_seconds", C.c_double),
                 ("scatter_seconds", C.c_double),
                 ("finish_seconds", C.c_double),
-                ("proj_kernel_seconds", C.c_double)]
+                ("proj_kernel_seconds", C.c_double),
+                ("early_dense", C.c_longlong),
+                ("tile_rounds", C.c_longlong)]
 
 
 class KernelDesc(C.Structure):
@@ -231,6 +233,39 @@ def approximate_batch(problems, maxrank, tol, onfly=0):
     for i, p in enumerate(problems):
         p.M = Ms[i] if info == 0 else None
     return info
+
+
+class Tuning(C.Structure):
+    """starsh_hip_tuning."""
+    _fields_ = [("form", C.c_int), ("first_bucket", C.c_int),
+                ("tile_schedule", C.c_int), ("qr_panel", C.c_int),
+                ("sketch_bn", C.c_int), ("sketch_rw", C.c_int),
+                ("proj_bn", C.c_int), ("proj_rw", C.c_int),
+                ("ntrials", C.c_int),
+                ("first_seconds", C.c_double * 5),
+                ("schedule_seconds", C.c_double),
+                ("untuned_seconds", C.c_double),
+                ("best_seconds", C.c_double)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["first_seconds"] = [x for x in self.first_seconds]
+        return d
+
+
+def autotune_batch(problems, maxrank, tol, form=1, onfly=0):
+    """starsh_hip_autotune_batch over TLRProblem objects (form 0 plain,
+    1 mixed); returns the Tuning record as a dict."""
+    n = len(problems)
+    mr = list(maxrank) if hasattr(maxrank, "__len__") else [maxrank] * n
+    tl = list(tol) if hasattr(tol, "__len__") else [tol] * n
+    Fs = (C.POINTER(BLRF) * n)(*[p.F for p in problems])
+    out = Tuning()
+    info = lib().starsh_hip_autotune_batch(
+        Fs, C.c_int(n), (C.c_int * n)(*mr), (C.c_double * n)(*tl),
+        C.c_int(onfly), C.c_int(form), C.byref(out))
+    assert info == 0, info
+    return out.as_dict()
 
 
 class TLRProblem:
