@@ -15,6 +15,9 @@
  *                                 buckets for the sketch width, FP64
  *                                 re-orthogonalisation, core SVD and checks.
  *
+ * starsh_blrm__dsdd_hip and starsh_blrm__dqp3_hip (STARSH_LRENGINE=SVD,
+ * DCSVD, RRQR) are the exact engines, below.
+ *
  * starsh_blrm__dfe_hip replaces src/backends/openmp/blrm/dfe.c:16-148.
  *
  * Tile contents are generated on the GPU from the problem's point data.  The
@@ -45,6 +48,27 @@ int starsh_blrm__drsdd_hip(struct starsh_blrm **matrix,
 
 /* Batched mixed-precision driver; same signature and output contract. */
 int starsh_blrm__drsdd_hip_mixed(struct starsh_blrm **matrix,
+        struct starsh_blrf *format, int maxrank, double tol, int onfly);
+
+/* Exact engines in FP64, same signature and output contract, selected by
+ * STARSH_LRENGINE=SVD or DCSVD (dsdd) and RRQR (dqp3) under either HIP
+ * backend:
+ *   starsh_blrm__dsdd_hip  replaces starsh_blrm__dsdd_omp
+ *                          (src/backends/openmp/blrm/dsdd.c) with
+ *                          starsh_dense_dlrsdd (sequential/dense/dsdd.c:
+ *                          16-66): the SVD of each far-field tile, rank by
+ *                          dsvfr over all singular values, low-rank when
+ *                          rank < min(m, n)/2 and rank <= maxrank;
+ *   starsh_blrm__dqp3_hip  replaces starsh_blrm__dqp3_omp with
+ *                          starsh_dense_dlrqp3 (sequential/dense/dqp3.c:
+ *                          16-93): column-pivoted QR, SVD of the leading
+ *                          min(maxrank+oversample, m, n) rows of R, the
+ *                          same rank rule over those singular values.
+ * A tile whose SVD core would exceed 1024 (more than 1024 singular values
+ * above 1e-5 tol |A|) is stored densely. */
+int starsh_blrm__dsdd_hip(struct starsh_blrm **matrix,
+        struct starsh_blrf *format, int maxrank, double tol, int onfly);
+int starsh_blrm__dqp3_hip(struct starsh_blrm **matrix,
         struct starsh_blrf *format, int maxrank, double tol, int onfly);
 
 /* Batch of independent problems in one call (no reference counterpart: the

@@ -22,8 +22,14 @@
  *   oracle_dsvfr          src/backends/sequential/dense/dsvfr.c:16-47.
  *   oracle_dlrrsdd        src/backends/sequential/dense/drsdd.c:16-93 (calls
  *                         dgemm, dgeqrf, dorgqr, dgesdd of the host LAPACK).
+ *   oracle_dlrsdd         src/backends/sequential/dense/dsdd.c:16-66 (dgesdd
+ *                         of the whole tile).
+ *   oracle_dlrqp3         src/backends/sequential/dense/dqp3.c:16-93 (dgeqp3,
+ *                         dorgqr, dgesdd of the leading rows of R).
  *   oracle_compress       src/backends/openmp/blrm/drsdd.c:94-145 (tile loop,
- *                         schedule(dynamic,1)) on a plain tiling.
+ *                         schedule(dynamic,1)) on a plain tiling; engine 1
+ *                         and 2 take the tile loops of openmp/blrm/dsdd.c
+ *                         and dqp3.c with the two functions above.
  *   oracle_dfe            src/backends/openmp/blrm/dfe.c:16-148.
  *   oracle_gen_*          src/applications/particles.c:177-201 and 341-411.
  *   oracle_zsort          src/applications/particles.c:1090-1221.
@@ -51,7 +57,11 @@ typedef void dgesdd_t(const char *, const int *, const int *, double *,
         const int *, double *, double *, const int *, double *, const int *,
         double *, const int *, int *, int *);
 
+typedef void dgeqp3_t(const int *, const int *, double *, const int *,
+        int *, double *, double *, const int *, int *);
+
 static dgemm_t *p_dgemm;
+static dgeqp3_t *p_dgeqp3;
 static dgeqrf_t *p_dgeqrf;
 static dorgqr_t *p_dorgqr;
 static dgesdd_t *p_dgesdd;
@@ -84,6 +94,8 @@ int oracle_lapack_load(const char *path)
             p_dorgqr = (dorgqr_t *)dlsym(h, nm);
             snprintf(nm, sizeof nm, "%sdgesdd_", prefixes[p]);
             p_dgesdd = (dgesdd_t *)dlsym(h, nm);
+            snprintf(nm, sizeof nm, "%sdgeqp3_", prefixes[p]);
+            p_dgeqp3 = (dgeqp3_t *)dlsym(h, nm);
             if(p_dgemm && p_dgeqrf && p_dorgqr && p_dgesdd)
             {
                 int (*layer)(int) = (int (*)(int))dlsym(h,
@@ -319,20 +331,124 @@ int oracle_dlrrsdd(int nrows, int ncols, const double *D, int ldD, double *U,
     return rank;
 }
 
+/* ---- dsdd.c:16-66.  Rank (-1 for a false far-field tile); U, V as for
+ * oracle_dlrrsdd; S (min(m, n) values) when not NULL.  D is overwritten. */
+int oracle_dlrsdd(int nrows, int ncols, double *D, int ldD, double *U,
+        int ldU, double *V, int ldV, int maxrank, double tol, double *S)
+{
+    int mn = nrows < ncols ? nrows : ncols;
+    double *sU = malloc(sizeof(double)*(size_t)nrows*mn);
+    double *sS = malloc(sizeof(double)*mn);
+    double *sV = malloc(sizeof(double)*(size_t)ncols*mn);
+    int *iwork = malloc(sizeof(int)*8*mn);
+    int lwork = -1, info;
+    double wq;
+    p_dgesdd("S", &nrows, &ncols, D, &ldD, sS, sU, &nrows, sV, &mn, &wq,
+            &lwork, iwork, &info);
+    lwork = (int)wq+1;
+    double *work = malloc(sizeof(double)*lwork);
+    p_dgesdd("S", &nrows, &ncols, D, &ldD, sS, sU, &nrows, sV, &mn, work,
+            &lwork, iwork, &info);
+    int rank = oracle_dsvfr(mn, sS, tol);
+    if(S)
+        memcpy(S, sS, sizeof(double)*mn);
+    if(info == 0 && rank < mn/2 && rank <= maxrank)
+    {
+        for(int i = 0; i < rank; i++)
+        {
+            if(U)
+                memcpy(U+(size_t)i*ldU, sU+(size_t)i*nrows,
+                        sizeof(double)*nrows);
+            if(V)
+                for(int j = 0; j < ncols; j++)
+                    V[(size_t)i*ldV+j] = sV[(size_t)j*mn+i]*sS[i];
+        }
+    }
+    else
+        rank = -1;
+    free(sU); free(sS); free(sV); free(iwork); free(work);
+    return rank;
+}
+
+/* ---- dqp3.c:16-93.  Rank (-1 for a false far-field tile); U, V as for
+ * oracle_dlrrsdd; S (mn2 values) and the column pivots (ncols, 0-based)
+ * when not NULL.  D is overwritten. */
+int oracle_dlrqp3(int nrows, int ncols, double *D, int ldD, double *U,
+        int ldU, double *V, int ldV, int maxrank, int oversample, double tol,
+        double *S, int *piv)
+{
+    int mn = nrows < ncols ? nrows : ncols;
+    int mn2 = maxrank+oversample;
+    if(mn2 > mn)
+        mn2 = mn;
+    int *jpvt = calloc(ncols, sizeof(int));
+    double *tau = malloc(sizeof(double)*mn);
+    double *R = calloc((size_t)ncols*mn2, sizeof(double));
+    double *sU = malloc(sizeof(double)*(size_t)mn2*mn2);
+    double *sS = malloc(sizeof(double)*mn2);
+    double *sV = malloc(sizeof(double)*(size_t)ncols*mn2);
+    int *iwork = malloc(sizeof(int)*8*mn2);
+    int lwork = -1, info;
+    double wq;
+    p_dgeqp3(&nrows, &ncols, D, &ldD, jpvt, tau, &wq, &lwork, &info);
+    int lw = (int)wq;
+    p_dorgqr(&nrows, &mn2, &mn2, D, &ldD, tau, &wq, &lwork, &info);
+    lw = (int)wq > lw ? (int)wq : lw;
+    p_dgesdd("S", &mn2, &ncols, R, &mn2, sS, sU, &mn2, sV, &mn2, &wq,
+            &lwork, iwork, &info);
+    lw = (int)wq > lw ? (int)wq : lw;
+    double *work = malloc(sizeof(double)*(lw+1));
+    p_dgeqp3(&nrows, &ncols, D, &ldD, jpvt, tau, work, &lw, &info);
+    /* R (mn2 x ncols) with the columns put back in their original order:
+     * column i of the factor is column jpvt[i]-1 of the tile */
+    for(int i = 0; i < ncols; i++)
+    {
+        const int j = jpvt[i]-1;
+        const int kk = i < mn2 ? i+1 : mn2;
+        for(int l = 0; l < kk; l++)
+            R[(size_t)j*mn2+l] = D[(size_t)i*ldD+l];
+    }
+    if(piv)
+        for(int i = 0; i < ncols; i++)
+            piv[i] = jpvt[i]-1;
+    p_dorgqr(&nrows, &mn2, &mn2, D, &ldD, tau, work, &lw, &info);
+    p_dgesdd("S", &mn2, &ncols, R, &mn2, sS, sU, &mn2, sV, &mn2, work, &lw,
+            iwork, &info);
+    int rank = oracle_dsvfr(mn2, sS, tol);
+    if(S)
+        memcpy(S, sS, sizeof(double)*mn2);
+    if(info == 0 && rank < mn/2 && rank <= maxrank)
+    {
+        const double one = 1.0, zero = 0.0;
+        if(rank > 0 && U)
+            p_dgemm("N", "N", &nrows, &rank, &mn2, &one, D, &ldD, sU, &mn2,
+                    &zero, U, &ldU);
+        if(V)
+            for(int i = 0; i < rank; i++)
+                for(int j = 0; j < ncols; j++)
+                    V[(size_t)i*ldV+j] = sV[(size_t)j*mn2+i]*sS[i];
+    }
+    else
+        rank = -1;
+    free(jpvt); free(tau); free(R); free(sU); free(sS); free(sV);
+    free(iwork); free(work);
+    return rank;
+}
+
 /* ---- openmp/blrm/drsdd.c:94-145 on a plain tiling with identity pivot.
  * Tile bi covers rows start[brow[bi]] .. +size and columns start[bcol[bi]]
  * .. +size of the point set(s).  U/V for tile bi go to U + uoff[bi] and
  * V + voff[bi] (maxrank columns each, ld = rows / cols) when U and V are
  * not NULL; S to S + bi*smax when S is not NULL. */
-int oracle_compress(const oracle_kernel *kd, const double *xr, int64_t cr,
-        const double *xc, int64_t cc, int64_t nfar, const int64_t *brow,
-        const int64_t *bcol, const int64_t *rstart, const int64_t *rsize,
-        const int64_t *cstart, const int64_t *csize, int maxrank,
-        int oversample, double tol, int nthreads, int *rank, double *U,
-        const int64_t *uoff, double *V, const int64_t *voff, double *S,
-        int smax)
+int oracle_compress_engine(int engine, const oracle_kernel *kd,
+        const double *xr, int64_t cr, const double *xc, int64_t cc,
+        int64_t nfar, const int64_t *brow, const int64_t *bcol,
+        const int64_t *rstart, const int64_t *rsize, const int64_t *cstart,
+        const int64_t *csize, int maxrank, int oversample, double tol,
+        int nthreads, int *rank, double *U, const int64_t *uoff, double *V,
+        const int64_t *voff, double *S, int smax)
 {
-    if(oracle_lapack_load(NULL) != 0)
+    if(oracle_lapack_load(NULL) != 0 || (engine == 2 && p_dgeqp3 == NULL))
         return -1;
     if(nthreads > 0)
         omp_set_num_threads(nthreads);
@@ -348,14 +464,35 @@ int oracle_compress(const oracle_kernel *kd, const double *xr, int64_t cr,
         for(int j = 0; j < n; j++)
             ic[j] = cstart[bcol[bi]]+j;
         oracle_kernel_block(kd, xr, cr, xc, cc, m, n, ir, ic, D, m);
-        rank[bi] = oracle_dlrrsdd(m, n, D, m, U ? U+uoff[bi] : NULL, m,
-                V ? V+voff[bi] : NULL, n, maxrank, oversample, tol,
-                S ? S+bi*(int64_t)smax : NULL);
+        double *Ub = U ? U+uoff[bi] : NULL, *Vb = V ? V+voff[bi] : NULL;
+        double *Sb = S ? S+bi*(int64_t)smax : NULL;
+        if(engine == 1)
+            rank[bi] = oracle_dlrsdd(m, n, D, m, Ub, m, Vb, n, maxrank, tol,
+                    Sb);
+        else if(engine == 2)
+            rank[bi] = oracle_dlrqp3(m, n, D, m, Ub, m, Vb, n, maxrank,
+                    oversample, tol, Sb, NULL);
+        else
+            rank[bi] = oracle_dlrrsdd(m, n, D, m, Ub, m, Vb, n, maxrank,
+                    oversample, tol, Sb);
         free(ir);
         free(ic);
         free(D);
     }
     return 0;
+}
+
+int oracle_compress(const oracle_kernel *kd, const double *xr, int64_t cr,
+        const double *xc, int64_t cc, int64_t nfar, const int64_t *brow,
+        const int64_t *bcol, const int64_t *rstart, const int64_t *rsize,
+        const int64_t *cstart, const int64_t *csize, int maxrank,
+        int oversample, double tol, int nthreads, int *rank, double *U,
+        const int64_t *uoff, double *V, const int64_t *voff, double *S,
+        int smax)
+{
+    return oracle_compress_engine(0, kd, xr, cr, xc, cc, nfar, brow, bcol,
+            rstart, rsize, cstart, csize, maxrank, oversample, tol, nthreads,
+            rank, U, uoff, V, voff, S, smax);
 }
 
 /* ---- openmp/blrm/dfe.c:16-148.  far tiles carry (brow, bcol, rank, U, V);

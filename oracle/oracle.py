@@ -105,9 +105,49 @@ def dlrrsdd(D, maxrank, oversample, tol):
     return r, U[:, :rr].copy(), V[:, :rr].copy(), S
 
 
+def dlrsdd(D, maxrank, tol):
+    """Returns (rank, U, V, S) of the reference dense SVD engine
+    (sequential/dense/dsdd.c:16-66) on D."""
+    assert lapack_load(), "no host LAPACK found"
+    D = np.array(D, dtype=np.float64, order="F")
+    m, n = D.shape
+    mn = min(m, n)
+    U = np.zeros((m, maxrank), order="F")
+    V = np.zeros((n, maxrank), order="F")
+    S = np.zeros(mn)
+    r = lib().oracle_dlrsdd(C.c_int(m), C.c_int(n), _d(D), C.c_int(m),
+                            _d(U), C.c_int(m), _d(V), C.c_int(n),
+                            C.c_int(maxrank), C.c_double(tol), _d(S))
+    rr = max(r, 0)
+    return r, U[:, :rr].copy(), V[:, :rr].copy(), S
+
+
+def dlrqp3(D, maxrank, oversample, tol):
+    """Returns (rank, U, V, S, pivots) of the reference dense RRQR engine
+    (sequential/dense/dqp3.c:16-93) on D; pivots are 0-based."""
+    assert lapack_load(), "no host LAPACK found"
+    D = np.array(D, dtype=np.float64, order="F")
+    m, n = D.shape
+    k = min(maxrank + oversample, m, n)
+    U = np.zeros((m, maxrank), order="F")
+    V = np.zeros((n, maxrank), order="F")
+    S = np.zeros(k)
+    piv = np.zeros(n, dtype=np.int32)
+    r = lib().oracle_dlrqp3(C.c_int(m), C.c_int(n), _d(D), C.c_int(m),
+                            _d(U), C.c_int(m), _d(V), C.c_int(n),
+                            C.c_int(maxrank), C.c_int(oversample),
+                            C.c_double(tol), _d(S), piv.ctypes.data_as(_ip))
+    rr = max(r, 0)
+    return r, U[:, :rr].copy(), V[:, :rr].copy(), S, piv.astype(np.int64)
+
+
+ENGINES = {"rsvd": 0, "sdd": 1, "rrqr": 2}
+
+
 def compress(kd, pts, starts, sizes, tiles, maxrank, oversample, tol,
-             nthreads=0, want_factors=True, want_S=False):
-    """Reference tile loop on plain tiling.  Returns dict with ranks and
+             nthreads=0, want_factors=True, want_S=False, engine="rsvd"):
+    """Reference tile loop on plain tiling (engine "rsvd": openmp/blrm/
+    drsdd.c, "sdd": dsdd.c, "rrqr": dqp3.c).  Returns dict with ranks and
     optionally U, V (lists) and S (ntiles x k)."""
     assert lapack_load(), "no host LAPACK found"
     pts = np.ascontiguousarray(pts, dtype=np.float64)
@@ -129,8 +169,11 @@ def compress(kd, pts, starts, sizes, tiles, maxrank, oversample, tol,
     U = np.zeros(max(su, 1)) if want_factors else None
     V = np.zeros(max(sv, 1)) if want_factors else None
     kmax = int(min(maxrank + oversample, sizes.max()))
+    if engine == "sdd":
+        kmax = int(sizes.max())
     S = np.zeros(nt * kmax) if want_S else None
-    info = lib().oracle_compress(
+    info = lib().oracle_compress_engine(
+        C.c_int(ENGINES[engine]),
         C.byref(kd), _d(pts), C.c_int64(ndata), _d(pts), C.c_int64(ndata),
         C.c_int64(nt), _l(brow), _l(bcol), _l(starts), _l(sizes), _l(starts),
         _l(sizes), C.c_int(maxrank), C.c_int(oversample), C.c_double(tol),

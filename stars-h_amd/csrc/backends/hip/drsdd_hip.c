@@ -622,7 +622,8 @@ static size_t tile_bytes(int m, int n, int k, int maxrank)
 int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
         const STARSH_int *idx, const int *k, int int8, double tau, double eta,
         int *rank_out, int *flags_out, int *accepted, hip_accept_fn accept,
-        double *S_out, int sstride, starsh_hip_stats *st)
+        double *S_out, int sstride, starsh_hip_stats *st,
+        const hip_run_extra *x)
 {
     if(ntiles == 0)
         return STARSH_SUCCESS;
@@ -637,8 +638,10 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
     double **du = malloc(sizeof(double *)*ntiles);
     double **dv = malloc(sizeof(double *)*ntiles);
     int64_t *so = malloc(sizeof(int64_t)*ntiles);
+    int64_t *xso = malloc(sizeof(int64_t)*ntiles);
+    int64_t *xpo = malloc(sizeof(int64_t)*ntiles);
     if(!m || !n || !pp || !kc || !r0 || !c0 || !uo || !vo || !probs || !du ||
-            !dv || !so)
+            !dv || !so || !xso || !xpo)
         info = STARSH_MALLOC_ERROR;
     for(STARSH_int t = 0; info == STARSH_SUCCESS && t < ntiles; t++)
         so[t] = (int64_t)t*sstride;
@@ -710,6 +713,27 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
             in.S_out = S_out;
             in.soff = so+b0;
         }
+        if(x)
+        {
+            /* offsets relative to the chunk's first tile */
+            in.rule = x->rule;
+            if(x->sel)
+            {
+                for(STARSH_int t = b0; t < b1; t++)
+                    xso[t] = x->sel_off[t]-x->sel_off[b0];
+                in.sel = x->sel+x->sel_off[b0];
+                in.sel_off = xso+b0;
+            }
+            if(x->piv_k)
+            {
+                for(STARSH_int t = b0; t < b1; t++)
+                    xpo[t] = x->piv_off[t]-x->piv_off[b0];
+                in.piv_k = x->piv_k+b0;
+                in.piv = x->piv+x->piv_off[b0];
+                in.piv_off = xpo+b0;
+                in.piv_done = x->piv_done+b0;
+            }
+        }
         RsddOutput out;
         memset(&out, 0, sizeof(out));
         out.rank = rank_out+b0;
@@ -736,7 +760,9 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
             int kfull = B->maxrank[q]+B->hp[q].oversample;
             kfull = kfull < mn ? kfull : mn;
             accepted[t] = accept ? accept(rank_out[t], kk, kfull,
-                    flags_out[t], B->hp[q].oversample) : 1;
+                    flags_out[t], B->hp[q].oversample, kc[t], mn) : 1;
+            if(x && x->kc_out)
+                x->kc_out[t] = kc[t];
         }
         /* finished tiles take the next blocks of their problem's host
          * buffers; their factors are queued for the copy-back, which runs
@@ -772,7 +798,8 @@ int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
         b0 = b1;
     }
     free(m); free(n); free(pp); free(kc); free(r0); free(c0); free(uo);
-    free(vo); free(probs); free(du); free(dv); free(so);
+    free(vo); free(probs); free(du); free(dv); free(so); free(xso);
+    free(xpo);
     return info;
 }
 
@@ -815,7 +842,7 @@ int hip_plain_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
         k[t] = B->maxrank[pidx[t]]+B->hp[pidx[t]].oversample;
     if(info == STARSH_SUCCESS)
         info = hip_driver_run(B, ntiles, pidx, idx, k, 0, 0.0, 0.0, rk, fl, acc,
-                NULL, NULL, 0, st);
+                NULL, NULL, 0, st, NULL);
     free(k); free(rk); free(fl); free(acc);
     return info;
 }
@@ -826,7 +853,7 @@ static int group_key(const hip_problem *hp)
     return hp->kd.family*16+hp->kd.ndim;
 }
 
-int hip_compress(int mixed, int nprob, STARSH_blrm **matrix,
+int hip_compress(int engine, int nprob, STARSH_blrm **matrix,
         STARSH_blrf **format, const int *maxrank, const double *tol,
         int onfly)
 {
@@ -870,7 +897,7 @@ int hip_compress(int mixed, int nprob, STARSH_blrm **matrix,
     }
     /* knobs recorded by the autotuner for this problem, if any */
     if(info == STARSH_SUCCESS)
-        hip_sched_apply(mixed, &B);
+        hip_sched_apply(engine == HIP_ENGINE_MIXED, &B);
     st.prepare_seconds = hip_driver_now()-t_start;
     /* all far-field tiles, grouped by kernel family and dimension */
     STARSH_int ntot = 0;
@@ -897,8 +924,13 @@ int hip_compress(int mixed, int nprob, STARSH_blrm **matrix,
                     idx[nt++] = bi;
                 }
             }
-        info = mixed ? hip_mixed_tiles(&B, nt, pidx, idx, &st) :
-            hip_plain_tiles(&B, nt, pidx, idx, &st);
+        if(engine == HIP_ENGINE_MIXED)
+            info = hip_mixed_tiles(&B, nt, pidx, idx, &st);
+        else if(engine == HIP_ENGINE_SDD || engine == HIP_ENGINE_RRQR)
+            info = hip_exact_tiles(&B, nt, pidx, idx,
+                    engine == HIP_ENGINE_RRQR, &st);
+        else
+            info = hip_plain_tiles(&B, nt, pidx, idx, &st);
     }
     free(pidx); free(idx); free(done);
     if(info == STARSH_SUCCESS)
@@ -978,6 +1010,30 @@ int starsh_blrm__drsdd_hip_mixed(STARSH_blrm **matrix, STARSH_blrf *format,
     return hip_compress(1, 1, matrix, &format, &maxrank, &tol, onfly);
 }
 
+int starsh_blrm__dsdd_hip(STARSH_blrm **matrix, STARSH_blrf *format,
+        int maxrank, double tol, int onfly)
+{
+    if(matrix == NULL || format == NULL)
+    {
+        STARSH_ERROR("Invalid value of `matrix` or `format`");
+        return STARSH_WRONG_PARAMETER;
+    }
+    return hip_compress(HIP_ENGINE_SDD, 1, matrix, &format, &maxrank, &tol,
+            onfly);
+}
+
+int starsh_blrm__dqp3_hip(STARSH_blrm **matrix, STARSH_blrf *format,
+        int maxrank, double tol, int onfly)
+{
+    if(matrix == NULL || format == NULL)
+    {
+        STARSH_ERROR("Invalid value of `matrix` or `format`");
+        return STARSH_WRONG_PARAMETER;
+    }
+    return hip_compress(HIP_ENGINE_RRQR, 1, matrix, &format, &maxrank, &tol,
+            onfly);
+}
+
 int starsh_blrm_approximate_batch(STARSH_blrm **matrix, STARSH_blrf **format,
         int nprob, const int *maxrank, const double *tol, int onfly)
 {
@@ -999,6 +1055,12 @@ int starsh_blrm_approximate_batch(STARSH_blrm **matrix, STARSH_blrf **format,
         return hip_compress(0, nprob, matrix, format, maxrank, tol, onfly);
     if(starsh_blrm_approximate == starsh_blrm__drsdd_hip_mixed)
         return hip_compress(1, nprob, matrix, format, maxrank, tol, onfly);
+    if(starsh_blrm_approximate == starsh_blrm__dsdd_hip)
+        return hip_compress(HIP_ENGINE_SDD, nprob, matrix, format, maxrank,
+                tol, onfly);
+    if(starsh_blrm_approximate == starsh_blrm__dqp3_hip)
+        return hip_compress(HIP_ENGINE_RRQR, nprob, matrix, format, maxrank,
+                tol, onfly);
     if(starsh_blrm_approximate == NULL)
     {
         STARSH_ERROR("starsh_init() was not called");

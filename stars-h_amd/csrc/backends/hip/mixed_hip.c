@@ -64,8 +64,11 @@ int hip_mixed_set_early_dense(double f)
  * error factor sqrt(1 + r/(p-1)) over the best rank-r approximation; the
  * reference keeps p = maxrank + oversample - r.  A bucket result is final
  * when p >= max(oversample, r), i.e. the factor stays below sqrt(2). */
-static int accept_mixed(int rank, int k, int kfull, int flags, int os)
+static int accept_mixed(int rank, int k, int kfull, int flags, int os,
+        int kc, int mn)
 {
+    (void)kc;
+    (void)mn;
     /* flag 4: the core SVD did not converge; the tile is a false far-field
      * tile (rank -1), as when dgesdd fails in drsdd.c:78 */
     if(k >= kfull || (flags & 4) != 0)
@@ -215,12 +218,13 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
             info = hip_driver_run(B, g1-g0, pq+g0, pi+g0, k+g0, int8, tau,
                     mixed_eta,
                     rk+g0, fl+g0, acc+g0, accept_mixed,
-                    sv+(size_t)g0*NBMAX, NBMAX, st);
+                    sv+(size_t)g0*NBMAX, NBMAX, st, NULL);
             g0 = g1;
         }
         /* optional per-tile record of the rounds (STARSH_HIP_MIXED_TRACE):
          * round, problem, tile, sketch width, rank, accepted, predicted
-         * rank, first singular values */
+         * rank, first and last singular values, Jacobi sweeps and size p
+         * of the core */
         FILE *tf = getenv("STARSH_HIP_MIXED_TRACE") ?
             fopen(getenv("STARSH_HIP_MIXED_TRACE"), round ? "a" : "w") : NULL;
         for(STARSH_int u = 0; tf && u < npend; u++)
@@ -228,9 +232,11 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
             const int q = pq[u], os = B->hp[q].oversample;
             const double pr = predicted_rank(sv+(size_t)u*NBMAX, k[u], os,
                     B->tol[q]);
-            fprintf(tf, "%d %d %lld %d %d %d %.3g %.3e %.3e\n", round, q,
+            fprintf(tf, "%d %d %lld %d %d %d %.3g %.3e %.3e %d %d\n", round,
+                    q,
                     (long long)pi[u], k[u], rk[u], acc[u], pr,
-                    sv[(size_t)u*NBMAX], sv[(size_t)u*NBMAX+k[u]-1]);
+                    sv[(size_t)u*NBMAX], sv[(size_t)u*NBMAX+k[u]-1],
+                    (fl[u] >> 8) & 255, fl[u] >> 16);
         }
         if(tf)
             fclose(tf);

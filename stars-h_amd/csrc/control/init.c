@@ -47,26 +47,19 @@ lrengine_table[LRENGINE_NUM] =
     {"CROSS", STARSH_LRENGINE_CROSS},
 };
 
-/* Engines without a HIP implementation land here. */
-static int unsupported_engine(STARSH_blrm **matrix, STARSH_blrf *format,
-        int maxrank, double tol, int onfly)
-{
-    (void)matrix; (void)format; (void)maxrank; (void)tol; (void)onfly;
-    STARSH_ERROR("low-rank engine %d is not implemented by the HIP backends "
-            "(use RSVD)", (int)starsh_params.lrengine);
-    return STARSH_WRONG_PARAMETER;
-}
-
-/* Row order SVD, DCSVD, RRQR, RSVD, CROSS as in init.h:111-190. */
+/* Row order SVD, DCSVD, RRQR, RSVD, CROSS as in init.h:111-190 (SVD and
+ * DCSVD both the divide-and-conquer SVD engine, CROSS the randomized SVD,
+ * as in the reference's tables).  The exact engines compute in FP64 in
+ * both HIP backends. */
 static STARSH_blrm_approximate *dlr_hip[LRENGINE_NUM] =
 {
-    unsupported_engine, unsupported_engine, unsupported_engine,
+    starsh_blrm__dsdd_hip, starsh_blrm__dsdd_hip, starsh_blrm__dqp3_hip,
     starsh_blrm__drsdd_hip, starsh_blrm__drsdd_hip
 };
 
 static STARSH_blrm_approximate *dlr_hip_mixed[LRENGINE_NUM] =
 {
-    unsupported_engine, unsupported_engine, unsupported_engine,
+    starsh_blrm__dsdd_hip, starsh_blrm__dsdd_hip, starsh_blrm__dqp3_hip,
     starsh_blrm__drsdd_hip_mixed, starsh_blrm__drsdd_hip_mixed
 };
 

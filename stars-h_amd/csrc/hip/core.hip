@@ -480,6 +480,13 @@ __global__ __launch_bounds__(NT) void core_kernel(TileDesc *tiles,
     TileDesc &td = tiles[blockIdx.x];
     const int p = td.p, k = td.k, maxrank = td.maxrank;
     const double tol = td.tol;
+    if(p < 0)
+    {
+        // core too large (pipeline.hip): a false far-field tile
+        if(!big && threadIdx.x == 0)
+            td.rank = -1;
+        return;
+    }
     if(big ? p <= LDS_PMAX : p > lds_p)
         return;
     if(p <= 0)
@@ -610,11 +617,17 @@ __global__ __launch_bounds__(NT) void core_kernel(TileDesc *tiles,
         // the sweep limit (the counterpart of info != 0 from dgesdd), marks
         // a false far-field tile (drsdd.c:78,89-92)
         const bool unconverged = sweeps >= max_sweeps;
-        if(r > maxrank || unconverged)
+        // the exact engines also require rank < min(m, n)/2 (dsdd.c:53,
+        // dqp3.c:84)
+        const int mn = td.m < td.n ? td.m : td.n;
+        if(r > maxrank || unconverged ||
+                ((td.rule & RSDD_RULE_HALF) && r >= mn/2))
             r = -1;
         s_rank = r;
         td.rank = r;
-        td.flags |= unconverged ? 4 : 0;
+        // bit 2: not converged; bits 8-15: sweeps, 16-26: p (diagnostics)
+        td.flags |= (unconverged ? 4 : 0) | (min(sweeps, 255) << 8) |
+            (p << 16);
     }
     __syncthreads();
     const int r = s_rank;

@@ -7,6 +7,7 @@
 #include "starsh.h"
 #include "starsh-hip.h"
 #include "besselk.h"
+#include "rsdd.h"
 
 #define HIP_TRY(expr) do { hipError_t e_ = (expr); if(e_ != hipSuccess) { \
     fprintf(stderr, "STARSH HIP ERROR: %s at %s:%d\n", hipGetErrorString(e_), \
@@ -93,6 +94,10 @@ struct TileDesc
     KernelParams kp;
     double tol;        // accuracy of the rank rule
     int maxrank;
+    int rule;          // RSDD_RULE_* bits
+    int64_t sel_off;   // selected columns (RSDD_RULE_SELECT), k ints
+    int64_t piv_off;   // column pivots of B (piv_k ints), -1: none
+    int piv_k;
 };
 
 __device__ inline double matern_closed(double nu, double x)

@@ -93,3 +93,12 @@ int pipeline_proj_i8(const TileDesc *d_tiles, int ntiles, int nmax, int kmax,
 // (each tile's own maxrank and tol) and X_r.
 int pipeline_core(TileDesc *d_tiles, int ntiles, double *ws, int pmax,
         hipStream_t s);
+
+// Exact engines (exact.hip): sketch Y = A[:, sel] (RSDD_RULE_SELECT tiles,
+// k columns from d_sel at each tile's sel_off), and the first piv_k column
+// pivots of B = Q^T A for tiles with kc > piv_k (Bt is overwritten).
+int pipeline_select(const TileDesc *d_tiles, int ntiles, int mmax, int kmax,
+        int ndim, double *ws, const int *d_sel, const double *abase,
+        hipStream_t s);
+int pipeline_qrcp(TileDesc *d_tiles, int ntiles, int nmax, int kmax,
+        double *ws, int *d_piv, hipStream_t s);

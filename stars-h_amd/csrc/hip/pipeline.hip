@@ -20,6 +20,8 @@ namespace {
 // Deflation thresholds of the Gram-Schmidt passes (relative to the column
 // norm before projection, and to 1 for the second pass).
 constexpr double TAU_FIRST = 1e-13;
+// Largest core SVD (pipeline_core).
+constexpr int CORE_PMAX = 1024;
 constexpr double TAU_SECOND = 1e-8;
 // Truncation of B's rows: discarded tail at most (ETA*tol)^2 |B|_F^2
 // (RsddInput.eta overrides it).
@@ -307,6 +309,10 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
         if(d.k < 1)
             d.k = 1;
         d.gen = in->d_a == nullptr;
+        d.rule = in->rule;
+        d.sel_off = in->sel_off ? in->sel_off[t] : 0;
+        d.piv_k = in->piv_k ? in->piv_k[t] : 0;
+        d.piv_off = d.piv_k > 0 && in->piv_off ? in->piv_off[t] : -1;
         d.r0 = in->r0 ? in->r0[t] : 0;
         d.c0 = in->c0 ? in->c0[t] : 0;
         d.a_off = in->a_off ? in->a_off[t] : 0;
@@ -317,7 +323,11 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     // Workspace layout: all Omega first, then per-tile regions.  The int8
     // sketch keeps its integer Omega in a separate byte buffer.
     int64_t off8 = 0;
-    for(int t = 0; t < nt; t++)
+    const bool select = (in->rule & RSDD_RULE_SELECT) != 0;
+    if(select && (in->sel == nullptr || in->sel_off == nullptr ||
+                in->sketch_int8))
+        return STARSH_WRONG_PARAMETER;
+    for(int t = 0; t < nt && !select; t++)
     {
         auto key = std::make_pair(h[t].n, h[t].k);
         if(omegas.find(key) == omegas.end())
@@ -367,8 +377,29 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     if(in->rows_out)
         for(int t = 0; t < nt; t++)
             in->rows_out[t] = h[t].k;
+    // selected columns (selection sketch) and column pivots (RRQR)
+    int64_t nsel = 0, npiv = 0;
+    for(int t = 0; t < nt; t++)
+    {
+        if(select)
+            nsel = std::max<int64_t>(nsel, h[t].sel_off+h[t].k);
+        if(h[t].piv_off >= 0)
+            npiv = std::max<int64_t>(npiv, h[t].piv_off+h[t].piv_k);
+    }
     // part: split-K partials of R2, kept until the batch has synchronised
-    DeviceBuf tiles, keep, norms, om8, active, qd, part;
+    DeviceBuf tiles, keep, norms, om8, active, qd, part, selbuf, pivbuf;
+    if(nsel > 0)
+    {
+        TRY(selbuf.alloc(sizeof(int)*(size_t)nsel));
+        HIP_TRY(hipMemcpyAsync(selbuf.p, in->sel, sizeof(int)*(size_t)nsel,
+                    hipMemcpyHostToDevice, s));
+    }
+    if(npiv > 0)
+    {
+        if(in->piv == nullptr)
+            return STARSH_WRONG_PARAMETER;
+        TRY(pivbuf.alloc(sizeof(int)*(size_t)npiv));
+    }
     TRY(ws_pool.reserve(sizeof(double)*off));
     if(in->sketch_int8)
         TRY(om8.alloc(off8));
@@ -421,7 +452,10 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     HIP_TRY(hipEventRecord(ev[0], s));
     // 1. sketch
     HIP_TRY(hipEventRecord(ev[2], s));
-    if(in->sketch_int8)
+    if(select)
+        TRY(pipeline_select(d_tiles, nt, mmax, kmax, g.ndim, d_ws,
+                    (const int *)selbuf.p, in->d_a, s));
+    else if(in->sketch_int8)
     {
         if(in->d_a != nullptr || probs[q0].kd == nullptr)
             return STARSH_WRONG_PARAMETER;
@@ -470,9 +504,24 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     HIP_TRY(hipMemcpyAsync(h.data(), d_tiles, sizeof(TileDesc)*nt,
                 hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    // The core SVD takes p <= CORE_PMAX; a tile whose truncated B is
+    // larger (only in the full-width rounds of the exact engines) has more
+    // than CORE_PMAX singular values above eta tol |B| and is stored
+    // densely (rank -1, flag 8): p = -1 skips it in the later stages.
     int pmax = 0;
+    bool capped = false;
     for(int t = 0; t < nt; t++)
-        pmax = h[t].p > pmax ? h[t].p : pmax;
+        if(h[t].p > CORE_PMAX)
+        {
+            h[t].p = -1;
+            h[t].flags |= 8;
+            capped = true;
+        }
+        else
+            pmax = h[t].p > pmax ? h[t].p : pmax;
+    if(capped)
+        HIP_TRY(hipMemcpyAsync(d_tiles, h.data(), sizeof(TileDesc)*nt,
+                    hipMemcpyHostToDevice, s));
     // 5. Bt[:, :p] = Q2 R2
     if(pmax > 0)
     {
@@ -499,6 +548,15 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
         TRY(pipeline_gemm(GM_FORM_U, d_tiles, nt, mmax, rmax, g, s));
         TRY(pipeline_gemm(GM_FORM_V, d_tiles, nt, nmax, rmax, g, s));
     }
+    // 8. column pivots of B for the tiles that need them (Bt is no longer
+    // used and is overwritten)
+    if(npiv > 0)
+    {
+        TRY(pipeline_qrcp(d_tiles, nt, nmax, kmax, d_ws, (int *)pivbuf.p,
+                    s));
+        HIP_TRY(hipMemcpyAsync(in->piv, pivbuf.p, sizeof(int)*(size_t)npiv,
+                    hipMemcpyDeviceToHost, s));
+    }
     HIP_TRY(hipEventRecord(ev[1], s));
     if(in->S_out)
         TRY(copy_segments(d_ws, h, in->S_out, in->soff, s));
@@ -517,6 +575,8 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
         out->p[t] = h[t].p;
         out->kc[t] = h[t].kc;
         out->flags[t] = h[t].flags;
+        if(in->piv_done)
+            in->piv_done[t] = h[t].piv_off >= 0 && h[t].kc > h[t].piv_k;
     }
     out->d_uv = (double *)uv_pool.p;
     return STARSH_SUCCESS;

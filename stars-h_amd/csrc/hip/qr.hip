@@ -324,7 +324,9 @@ __global__ __launch_bounds__(NT) void bcgs_kernel(TileDesc *tiles, int which,
                 }
             }
         }
-        if(pass == 0 && ndropped == jb)
+        // (a selection sketch A[:, sel] is not a random combination of the
+        // columns: a dropped panel does not mean its range is exhausted)
+        if(pass == 0 && ndropped == jb && !(td.rule & RSDD_RULE_SELECT))
         {
             // the range is exhausted: the panel is zero, later panels are
             // skipped.  Q2 is used in place (no compaction), so the

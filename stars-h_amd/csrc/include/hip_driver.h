@@ -66,9 +66,27 @@ typedef struct hip_batch
 } hip_batch;
 
 /* Decides whether a tile's result is final: (rank, sketch width used, full
- * sketch width of the plain form, pipeline flags, oversample). */
+ * sketch width of the plain form, pipeline flags, oversample, kept sketch
+ * columns, min(m, n)). */
 typedef int (*hip_accept_fn)(int rank, int k, int kfull, int flags,
-        int oversample);
+        int oversample, int kc, int mn);
+
+/* Inputs of the exact engines for hip_driver_run (per tile of the call):
+ * rule (RSDD_RULE_* bits), selected columns sel[sel_off[t] ...] (k[t] of
+ * them, RSDD_RULE_SELECT), and pivots: the first piv_k[t] column pivots
+ * of B for tiles with more kept columns go to piv[piv_off[t] ...] with
+ * piv_done[t] = 1.  Offsets increase with t. */
+typedef struct hip_run_extra
+{
+    int rule;
+    const int *sel;
+    const int64_t *sel_off;
+    const int *piv_k;
+    int *piv;
+    const int64_t *piv_off;
+    int *piv_done;
+    int *kc_out;            /* kept sketch columns per tile, or NULL */
+} hip_run_extra;
 
 /* Compresses far-field tiles idx[t] of problems pidx[t] (t < ntiles, all of
  * one kernel family and dimension) with sketch widths k[] in
@@ -79,7 +97,8 @@ typedef int (*hip_accept_fn)(int rank, int k, int kfull, int flags,
 int hip_driver_run(const hip_batch *B, STARSH_int ntiles, const int *pidx,
         const STARSH_int *idx, const int *k, int int8, double tau, double eta,
         int *rank_out, int *flags_out, int *accepted, hip_accept_fn accept,
-        double *S_out, int sstride, starsh_hip_stats *st);
+        double *S_out, int sstride, starsh_hip_stats *st,
+        const hip_run_extra *x);
 void hip_driver_rank_stats(const hip_batch *B, starsh_hip_stats *st);
 /* Bound on the factor doubles and segments of a batch of problems. */
 void hip_driver_factor_bound(const hip_batch *B, int64_t *doubles,
@@ -94,6 +113,13 @@ int hip_plain_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
         const STARSH_int *idx, starsh_hip_stats *st);
 int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
         const STARSH_int *idx, starsh_hip_stats *st);
+/* The exact engines (exact_hip.c): SVD (rrqr == 0) or rank-revealing QR. */
+int hip_exact_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
+        const STARSH_int *idx, int rrqr, starsh_hip_stats *st);
+
+/* Engines of hip_compress. */
+enum { HIP_ENGINE_RSVD = 0, HIP_ENGINE_MIXED = 1, HIP_ENGINE_SDD = 2,
+    HIP_ENGINE_RRQR = 3 };
 
 /* Whole compression of nprob problems (plain form when mixed == 0):
  * prepare, compress every far-field tile, finish each problem. */

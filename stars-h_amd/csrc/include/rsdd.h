@@ -51,7 +51,27 @@ typedef struct rsdd_input
     double *S_out;               /* optional: k singular values per tile */
     const int64_t *soff;
     int *rows_out;               /* optional: sketch width k per tile */
+    /* Exact engines (SDD, RRQR): RSDD_RULE_HALF also declares a tile
+     * false far-field when 2 rank >= min(m, n) (dsdd.c:53, dqp3.c:84);
+     * RSDD_RULE_SELECT replaces the random sketch by the k columns
+     * sel[sel_off[t] ...] of the tile (Y = A[:, sel]) and keeps
+     * orthogonalising after a fully dropped panel. */
+    int rule;
+    const int *sel;
+    const int64_t *sel_off;
+    /* Optional: for every tile with more than piv_k[t] > 0 kept sketch
+     * columns, the first piv_k[t] column pivots of a column-pivoted QR of
+     * B = Q^T A (greedy largest residual norm, the pivot order of
+     * dgeqp3), written to piv[piv_off[t] ...]; piv_done[t] = 1 when they
+     * were computed. */
+    const int *piv_k;
+    int *piv;
+    const int64_t *piv_off;
+    int *piv_done;
 } RsddInput;
+
+#define RSDD_RULE_HALF 1
+#define RSDD_RULE_SELECT 2
 
 typedef struct rsdd_output
 {

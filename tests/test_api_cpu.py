@@ -73,14 +73,30 @@ def test_init_reads_environment(monkeypatch):
     assert list(params) == [9, 3, 10]
 
 
-def test_unsupported_engine_reports_error_without_gpu():
+@pytest.mark.parametrize("backend", [b"HIP", b"HIP_MIXED"])
+def test_engine_table(backend):
+    """STARSH_LRENGINE selects the engine as the reference's tables do
+    (include/control/init.h:111-190): SVD and DCSVD the SVD engine, RRQR
+    the pivoted-QR engine, RSVD and CROSS the randomized SVD (the mixed
+    form under HIP_MIXED)."""
     L = S.lib()
-    assert L.starsh_set_backend(b"HIP") == 0
-    assert L.starsh_set_lrengine(b"SVD") == 0
-    fn = S.approximate_fn()
-    p = S.TLRProblem("spatial", 400, 2, 100, seed=0, kernel="exp")
-    assert p.approximate(20, 1e-6, 0, fn) == S.STARSH_WRONG_PARAMETER
-    assert L.starsh_set_lrengine(b"RSVD") == 0
+    addr = {name: C.cast(getattr(L, name), C.c_void_p).value
+            for name in ("starsh_blrm__dsdd_hip", "starsh_blrm__dqp3_hip",
+                         "starsh_blrm__drsdd_hip",
+                         "starsh_blrm__drsdd_hip_mixed")}
+    rsvd = ("starsh_blrm__drsdd_hip_mixed" if backend == b"HIP_MIXED" else
+            "starsh_blrm__drsdd_hip")
+    want = {b"SVD": "starsh_blrm__dsdd_hip", b"DCSVD": "starsh_blrm__dsdd_hip",
+            b"RRQR": "starsh_blrm__dqp3_hip", b"RSVD": rsvd, b"CROSS": rsvd}
+    assert L.starsh_set_backend(backend) == 0
+    try:
+        for eng, name in want.items():
+            assert L.starsh_set_lrengine(eng) == 0
+            fp = C.c_void_p.in_dll(L, "starsh_blrm_approximate").value
+            assert fp == addr[name], eng
+    finally:
+        assert L.starsh_set_backend(b"HIP") == 0
+        assert L.starsh_set_lrengine(b"RSVD") == 0
 
 
 @pytest.mark.parametrize("place,ndim,seed", [(6, 2, 0), (6, 3, 1), (1, 2, 5),
