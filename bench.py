@@ -148,11 +148,11 @@ def _timed(S, w, fn, steps):
     return out
 
 
-def run_form(S, w, backend, steps, warmup, autotune):
-    fn = S.init(backend, oversample=w["os"])
-    mixed = backend == "HIP_MIXED"
+def run_form(S, w, backend, steps, warmup, autotune, lrengine="RSVD"):
+    fn = S.init(backend, oversample=w["os"], lrengine=lrengine)
+    mixed = backend == "HIP_MIXED" and lrengine == "RSVD"
     schedule = None
-    if autotune:
+    if autotune and lrengine == "RSVD":
         # plan step (outside the timed steps): knobs by trial
         # (starsh_hip_autotune_batch; all problems of C5 as one batch)
         t = S.autotune_batch(make_problems(S, w), w["maxrank"], w["tol"],
@@ -309,7 +309,10 @@ def main():
                          "for C2, 256 otherwise)")
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="plain,mixed")
+    ap.add_argument("--forms", default="plain,mixed",
+                    help="comma-separated: plain, mixed (randomized SVD), "
+                         "sdd, rrqr (exact engines, STARSH_LRENGINE=SVD / "
+                         "RRQR)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the FP64 rate probe and the product timing "
@@ -349,14 +352,19 @@ def main():
             b"mixed_first_bucket", ctypes.c_double(args.first_bucket)) == 0
     forms = {}
     for form in args.forms.split(","):
-        backend = {"plain": "HIP", "mixed": "HIP_MIXED"}[form]
+        backend, lre = {"plain": ("HIP", "RSVD"),
+                        "mixed": ("HIP_MIXED", "RSVD"),
+                        "sdd": ("HIP", "SVD"),
+                        "rrqr": ("HIP", "RRQR")}[form]
         print(f"bench: {args.config} {form} form, {args.warmup} warmup + "
               f"{args.steps} steps", file=sys.stderr, flush=True)
         forms[form] = run_form(S, w, backend, args.steps, args.warmup,
-                               not args.no_autotune)
+                               not args.no_autotune, lre)
+    S.init("HIP", lrengine="RSVD")
     plain = forms.get("plain")
-    head = forms.get("mixed") or plain
-    head_name = "mixed" if "mixed" in forms else "plain"
+    head_name = ("mixed" if "mixed" in forms else
+                 "plain" if "plain" in forms else next(iter(forms)))
+    head = forms[head_name]
     cpu = None
     if not args.no_cpu and plain is not None and w["kind"] != "batch":
         print("bench: CPU baseline sample", file=sys.stderr, flush=True)
@@ -434,7 +442,7 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64" if head_name == "plain" else "i8+f64",
+        "dtype": "i8+f64" if head_name == "mixed" else "f64",
         "data": "synthetic (seeded point sets, glibc rand / numpy)",
         "config": {"workload": w["desc"], "form": head_name},
         "forms": summary,
