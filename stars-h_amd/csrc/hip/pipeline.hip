@@ -27,23 +27,6 @@ constexpr double TAU_SECOND = 1e-8;
 // (RsddInput.eta overrides it).
 constexpr double ETA = 1e-5;
 
-struct DeviceBuf
-{
-    void *p = nullptr;
-    ~DeviceBuf() { if(p) (void)hipFree(p); }
-    int alloc(size_t bytes)
-    {
-        if(hipMalloc(&p, bytes ? bytes : 1) != hipSuccess)
-        {
-            p = nullptr;
-            fprintf(stderr, "STARSH HIP ERROR: hipMalloc(%zu) failed\n",
-                    bytes);
-            return STARSH_MALLOC_ERROR;
-        }
-        return STARSH_SUCCESS;
-    }
-};
-
 #define TRY(expr) do { int i_ = (expr); if(i_ != STARSH_SUCCESS) return i_; } \
     while(0)
 
@@ -54,13 +37,14 @@ struct Pool
     void *p = nullptr;
     size_t bytes = 0;
     bool host = false;
+    // allocations are rounded up to this to absorb small variations
+    // between calls
+    size_t unit = (size_t)64 << 20;
     int reserve(size_t want)
     {
         if(want <= bytes && p)
             return STARSH_SUCCESS;
         release();
-        // round up to 64 MiB to absorb small variations between calls
-        const size_t unit = (size_t)64 << 20;
         size_t b = (want+unit-1)/unit*unit;
         hipError_t e = host ? hipHostMalloc(&p, b, hipHostMallocDefault) :
             hipMalloc(&p, b);
@@ -90,6 +74,20 @@ struct Pool
 };
 
 Pool ws_pool, uv_pool, pack_pool, stage_pool{nullptr, 0, true};
+// The small per-batch buffers (tile descriptors, kept-column flags, norms,
+// int8 Omega and digit planes, counters, R2 partials, selected columns,
+// pivots), kept too: a hipMalloc / hipFree pair per buffer and batch cost
+// host time on every call, and hipFree waits for the device.
+constexpr size_t SMALL_UNIT = (size_t)1 << 20;
+Pool tiles_pool{nullptr, 0, false, SMALL_UNIT},
+     keep_pool{nullptr, 0, false, SMALL_UNIT},
+     norms_pool{nullptr, 0, false, SMALL_UNIT},
+     om8_pool{nullptr, 0, false, SMALL_UNIT},
+     active_pool{nullptr, 0, false, SMALL_UNIT},
+     qd_pool{nullptr, 0, false, SMALL_UNIT},
+     part_pool{nullptr, 0, false, SMALL_UNIT},
+     sel_pool{nullptr, 0, false, SMALL_UNIT},
+     piv_pool{nullptr, 0, false, SMALL_UNIT};
 
 // Host view of the device counter of tiles still in a BCGS2 run.  The
 // counter is copied to pinned memory after every panel; the host reads it
@@ -387,10 +385,12 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
             npiv = std::max<int64_t>(npiv, h[t].piv_off+h[t].piv_k);
     }
     // part: split-K partials of R2, kept until the batch has synchronised
-    DeviceBuf tiles, keep, norms, om8, active, qd, part, selbuf, pivbuf;
+    Pool &tiles = tiles_pool, &keep = keep_pool, &norms = norms_pool,
+         &om8 = om8_pool, &active = active_pool, &qd = qd_pool,
+         &part = part_pool, &selbuf = sel_pool, &pivbuf = piv_pool;
     if(nsel > 0)
     {
-        TRY(selbuf.alloc(sizeof(int)*(size_t)nsel));
+        TRY(selbuf.reserve(sizeof(int)*(size_t)nsel));
         HIP_TRY(hipMemcpyAsync(selbuf.p, in->sel, sizeof(int)*(size_t)nsel,
                     hipMemcpyHostToDevice, s));
     }
@@ -398,18 +398,18 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     {
         if(in->piv == nullptr)
             return STARSH_WRONG_PARAMETER;
-        TRY(pivbuf.alloc(sizeof(int)*(size_t)npiv));
+        TRY(pivbuf.reserve(sizeof(int)*(size_t)npiv));
     }
     TRY(ws_pool.reserve(sizeof(double)*off));
     if(in->sketch_int8)
-        TRY(om8.alloc(off8));
+        TRY(om8.reserve(off8));
     if(proj8)
-        TRY(qd.alloc(offq));
-    TRY(tiles.alloc(sizeof(TileDesc)*nt));
-    TRY(keep.alloc(sizeof(int)*(size_t)nt*kmax));
-    TRY(norms.alloc(sizeof(double)*(size_t)nt*kmax));
+        TRY(qd.reserve(offq));
+    TRY(tiles.reserve(sizeof(TileDesc)*nt));
+    TRY(keep.reserve(sizeof(int)*(size_t)nt*kmax));
+    TRY(norms.reserve(sizeof(double)*(size_t)nt*kmax));
     TRY(uv_pool.reserve(sizeof(double)*(uvoff > 0 ? uvoff : 1)));
-    TRY(active.alloc(sizeof(int)));
+    TRY(active.reserve(sizeof(int)));
     TileDesc *d_tiles = (TileDesc *)tiles.p;
     double *d_ws = (double *)ws_pool.p;
     HIP_TRY(hipMemcpyAsync(d_tiles, h.data(), sizeof(TileDesc)*nt,
@@ -530,7 +530,7 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
         dump("Q2", d_ws+h[0].q2_off, (int64_t)h[0].n*h[0].p, s);
         const int nz = pipeline_r2_split(nt, pmax, nmax);
         if(nz > 1)
-            TRY(part.alloc(sizeof(double)*(size_t)nt*nz*pmax*pmax));
+            TRY(part.reserve(sizeof(double)*(size_t)nt*nz*pmax*pmax));
         TRY(pipeline_r2(d_tiles, nt, pmax, nz, (double *)part.p, g, s));
         dump("R2", d_ws+h[0].r_off, (int64_t)h[0].k*h[0].k, s);
     }
@@ -590,6 +590,9 @@ void rsdd_release_pools(void)
     uv_pool.release();
     pack_pool.release();
     stage_pool.release();
+    for(Pool *q : {&tiles_pool, &keep_pool, &norms_pool, &om8_pool,
+                &active_pool, &qd_pool, &part_pool, &sel_pool, &piv_pool})
+        q->release();
     rsdd_out_release();
 }
 
