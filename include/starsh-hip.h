@@ -64,8 +64,9 @@ int starsh_blrm__drsdd_hip_mixed(struct starsh_blrm **matrix,
  *                          16-93): column-pivoted QR, SVD of the leading
  *                          min(maxrank+oversample, m, n) rows of R, the
  *                          same rank rule over those singular values.
- * A tile whose SVD core would exceed 1024 (more than 1024 singular values
- * above 1e-5 tol |A|) is stored densely. */
+ * A tile with more than 1024 singular values above 1e-5 tol |A| keeps the
+ * 1024 leading rows of B = Q^T A for its SVD core and counts the others in
+ * the rank rule's tail (a conservative rank). */
 int starsh_blrm__dsdd_hip(struct starsh_blrm **matrix,
         struct starsh_blrf *format, int maxrank, double tol, int onfly);
 int starsh_blrm__dqp3_hip(struct starsh_blrm **matrix,

@@ -26,10 +26,11 @@
  *       same residual norms), and round 3 runs the pipeline with the sketch
  *       Y = A[:, pivots] (rank by dsvfr over mn2 values, dqp3.c:83-84).
  *
- * A tile whose truncated core would exceed 1024 (more than 1024 singular
- * values above 1e-5 tol |A|, only possible in round 2) is stored densely
- * (rank -1): accuracy is kept, the reference may find a rank below maxrank
- * there. */
+ * A tile whose truncated core would exceed 1024 rows (more than 1024
+ * singular values above 1e-5 tol |A|, only possible in round 2) keeps
+ * 1024 rows of B and counts the others in the rank rule's tail, like the
+ * truncation itself: the rank can then exceed dgesdd's by the margin that
+ * extra mass allows, never fall below it. */
 #include "common.h"
 #include "rsdd.h"
 #include "hip_driver.h"

@@ -480,13 +480,6 @@ __global__ __launch_bounds__(NT) void core_kernel(TileDesc *tiles,
     TileDesc &td = tiles[blockIdx.x];
     const int p = td.p, k = td.k, maxrank = td.maxrank;
     const double tol = td.tol;
-    if(p < 0)
-    {
-        // core too large (pipeline.hip): a false far-field tile
-        if(!big && threadIdx.x == 0)
-            td.rank = -1;
-        return;
-    }
     if(big ? p <= LDS_PMAX : p > lds_p)
         return;
     if(p <= 0)

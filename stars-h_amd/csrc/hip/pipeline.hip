@@ -504,21 +504,31 @@ int rsdd_batch(const RsddInput *in, RsddOutput *out)
     HIP_TRY(hipMemcpyAsync(h.data(), d_tiles, sizeof(TileDesc)*nt,
                 hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    // The core SVD takes p <= CORE_PMAX; a tile whose truncated B is
-    // larger (only in the full-width rounds of the exact engines) has more
-    // than CORE_PMAX singular values above eta tol |B| and is stored
-    // densely (rank -1, flag 8): p = -1 skips it in the later stages.
+    // The core SVD takes p <= CORE_PMAX.  A tile whose truncated B is
+    // larger (only in the full-width rounds of the exact engines) keeps its
+    // first CORE_PMAX rows; the squared norms of the rows dropped are added
+    // to its tail, as for the truncation itself, so the rank rule stays
+    // conservative (flag 8).
     int pmax = 0;
     bool capped = false;
     for(int t = 0; t < nt; t++)
+    {
         if(h[t].p > CORE_PMAX)
         {
-            h[t].p = -1;
+            std::vector<double> nb(h[t].p-CORE_PMAX);
+            HIP_TRY(hipMemcpy(nb.data(), (double *)norms.p+(int64_t)t*kmax+
+                        CORE_PMAX, sizeof(double)*nb.size(),
+                        hipMemcpyDeviceToHost));
+            double extra = 0.0;
+            for(size_t c = nb.size(); c > 0; c--)
+                extra += nb[c-1];
+            h[t].tail2 += extra;
+            h[t].p = CORE_PMAX;
             h[t].flags |= 8;
             capped = true;
         }
-        else
-            pmax = h[t].p > pmax ? h[t].p : pmax;
+        pmax = h[t].p > pmax ? h[t].p : pmax;
+    }
     if(capped)
         HIP_TRY(hipMemcpyAsync(d_tiles, h.data(), sizeof(TileDesc)*nt,
                     hipMemcpyHostToDevice, s));

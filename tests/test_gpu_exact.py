@@ -130,3 +130,48 @@ def test_rrqr_pivoted_tiles(backend):
                     seed=4, kernel="exp", beta=0.2)
     # some tiles went through the third (pivot-column) round
     assert st.tile_rounds > st.ntiles
+
+
+def test_sdd_matern3d_131072_sampled():
+    """The SVD engine at the 3D Matern workload's full size (N=131072, tile
+    2048, maxrank 512, oversample 16, eps 1e-7): error bound over all tiles,
+    and the ranks of sampled tiles against dgesdd in the oracle: the four
+    highest-rank tiles, two false far-field tiles and four spread tiles."""
+    tol, maxrank, os_ = 1e-7, 512, 16
+    fn = S.init("HIP", oversample=os_, lrengine="SVD")
+    try:
+        p = S.TLRProblem("spatial", 131072, 3, 2048, symm="S", seed=2,
+                         place=S.STARSH_PARTICLES_RAND, kernel="matern",
+                         beta=0.1, nu=1.5)
+        tiles0 = p.tiles()
+        _say("case: SVD engine, 3D Matern N=131072 nb=2048 maxrank 512")
+        assert p.approximate(maxrank, tol, 0, fn) == 0
+        st = S.stats()
+        got = dict(zip(p.tiles(), p.ranks().tolist()))
+        err = p.dfe()
+        _say(f"  device {st.device_seconds:.3f} s, rounds {st.tile_rounds}, "
+             f"false far {len(tiles0) - len(got)}, error {err:.3e}")
+        assert err <= 10 * tol
+        by_rank = sorted(got, key=lambda t: -got[t])
+        dense = [t for t in tiles0 if t not in got]
+        spread = [tiles0[i] for i in
+                  np.linspace(0, len(tiles0) - 1, 4).astype(int)]
+        pick = list(dict.fromkeys(by_rank[:4] + dense[:2] + spread))
+        kd = O.kernel("matern", 3, beta=0.1, nu=1.5)
+        ss, sz = p.cluster_ranges()
+        ref = O.compress(kd, p.points(), ss, sz, pick, maxrank, os_, tol,
+                         want_factors=False, want_S=True, engine="sdd")
+        for q, t in enumerate(pick):
+            rr, rg = int(ref["rank"][q]), got.get(t, -1)
+            s = ref["S"][q]
+            tot = np.sum(s * s)
+            _say(f"  tile {t}: gpu {rg} oracle {rr}")
+            if rr == rg:
+                continue
+            # a near tie of the rule at either rank
+            ties = [abs(np.sum(s[r:] ** 2) - tol * tol * tot) <=
+                    1e-6 * tol * tol * tot for r in (rr, rg) if r >= 0]
+            assert any(ties), (t, rg, rr)
+        p.free_matrix()
+    finally:
+        S.init("HIP", lrengine="RSVD")
