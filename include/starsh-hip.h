@@ -372,6 +372,17 @@ int starsh_hip_dense_rsdd(int ntiles, const int *m, const int *n,
         const int64_t *uoff, double *V, const int64_t *voff, double *S,
         const int64_t *soff);
 
+/* The same for the three engines: engine 0 randomized SVD (as above), 1
+ * SVD (starsh_dense_dlrsdd: all singular values, rank < min(m, n)/2 and
+ * rank <= maxrank), 2 rank-revealing QR (starsh_dense_dlrqp3); S + soff[t]
+ * receives the singular values of the tile's last round (up to min(m, n)
+ * for engine 1, min(maxrank+oversample, m, n) otherwise). */
+int starsh_hip_dense_lowrank(int engine, int ntiles, const int *m,
+        const int *n, const double *d_tiles, const int64_t *off, int maxrank,
+        int oversample, double tol, int *rank, double *U,
+        const int64_t *uoff, double *V, const int64_t *voff, double *S,
+        const int64_t *soff);
+
 #ifdef __cplusplus
 }
 #endif

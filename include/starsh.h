@@ -236,6 +236,21 @@ int starsh_itersolvers__dcg_omp(STARSH_blrm *matrix, int nrhs, double *B,
  * by relative Frobenius tail. */
 int starsh_dense_dsvfr(int size, double *S, double tol);
 
+/* The reference's dense engines for one tile (include/starsh.h:644-652,
+ * src/backends/sequential/dense/{dsdd,drsdd,dqp3}.c), same signatures and
+ * results, computed on the GPU: D (nrows x ncols, ld ldD) is copied to the
+ * device, U (ld ldU) and V (ld ldV) receive rank columns when the tile is
+ * low-rank, rank = -1 otherwise.  work, lwork and iwork are not used. */
+void starsh_dense_dlrsdd(int nrows, int ncols, double *D, int ldD, double *U,
+        int ldU, double *V, int ldV, int *rank, int maxrank, double tol,
+        double *work, int lwork, int *iwork);
+void starsh_dense_dlrrsdd(int nrows, int ncols, double *D, int ldD, double *U,
+        int ldU, double *V, int ldV, int *rank, int maxrank, int oversample,
+        double tol, double *work, int lwork, int *iwork);
+void starsh_dense_dlrqp3(int nrows, int ncols, double *D, int ldD, double *U,
+        int ldU, double *V, int ldV, int *rank, int maxrank, int oversample,
+        double tol, double *work, int lwork, int *iwork);
+
 #ifdef __cplusplus
 }
 #endif

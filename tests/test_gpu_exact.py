@@ -175,3 +175,62 @@ def test_sdd_matern3d_131072_sampled():
         p.free_matrix()
     finally:
         S.init("HIP", lrengine="RSVD")
+
+
+def _spectrum_tile(m, n, decay, seed):
+    rng = np.random.default_rng(seed)
+    mn = min(m, n)
+    Uq, _ = np.linalg.qr(rng.standard_normal((m, mn)))
+    Vq, _ = np.linalg.qr(rng.standard_normal((n, mn)))
+    D = (Uq * decay ** np.arange(mn)) @ Vq.T
+    return D * np.linspace(1.0, 2.0, n)[None, :]   # distinct column norms
+
+
+@pytest.mark.parametrize("engine", ["sdd", "rrqr", "rsvd"])
+def test_reference_dense_engines_on_host_tiles(engine):
+    """starsh_dense_dlrsdd / dlrqp3 / dlrrsdd (the reference's dense
+    entry points, host arrays with padded leading dimensions) against the
+    oracle's dense engines: same rank, U V^T within (1e-9 + 1e-4 tol) |D|
+    of the oracle's, rank -1 where the oracle's is."""
+    import ctypes as C
+    L = S.lib()
+    dp = C.POINTER(C.c_double)
+    # low rank; rank >= min(m, n)/2; wide; range beyond maxrank+oversample
+    # (the RRQR pivots decide) with a loose tolerance
+    cases = [(300, 260, 0.6, 60, 10, 1e-7), (200, 200, 0.97, 150, 10, 1e-6),
+             (180, 320, 0.5, 40, 8, 1e-9), (256, 256, 0.85, 30, 10, 1e-2)]
+    for m, n, decay, maxrank, os_, tol in cases:
+        _say(f"case: dense {engine} {m}x{n} decay {decay} maxrank {maxrank}")
+        D = _spectrum_tile(m, n, decay, seed=m + n)
+        ldd, ldu, ldv = m + 3, m + 5, n + 7
+        Dp = np.zeros((ldd, n), order="F")
+        Dp[:m] = D
+        Up = np.zeros((ldu, maxrank), order="F")
+        Vp = np.zeros((ldv, maxrank), order="F")
+        rank = C.c_int(-5)
+        args = [C.c_int(m), C.c_int(n), Dp.ctypes.data_as(dp), C.c_int(ldd),
+                Up.ctypes.data_as(dp), C.c_int(ldu), Vp.ctypes.data_as(dp),
+                C.c_int(ldv), C.byref(rank), C.c_int(maxrank)]
+        if engine == "sdd":
+            L.starsh_dense_dlrsdd(*args, C.c_double(tol), None, 0, None)
+            r_ref, Ur, Vr, _ = O.dlrsdd(D, maxrank, tol)
+        elif engine == "rrqr":
+            L.starsh_dense_dlrqp3(*args, C.c_int(os_), C.c_double(tol), None,
+                                  0, None)
+            r_ref, Ur, Vr, _, _ = O.dlrqp3(D, maxrank, os_, tol)
+        else:
+            L.starsh_dense_dlrrsdd(*args, C.c_int(os_), C.c_double(tol),
+                                   None, 0, None)
+            r_ref, Ur, Vr, _ = O.dlrrsdd(D, maxrank, os_, tol)
+        _say(f"  rank gpu {rank.value} oracle {r_ref}")
+        assert rank.value == r_ref
+        if (m, n) != (200, 200):
+            assert r_ref > 0      # the low-rank cases are low-rank
+        assert np.array_equal(Dp[:m], D)       # D is not modified
+        if r_ref > 0:
+            Ag = Up[:m, :r_ref] @ Vp[:n, :r_ref].T
+            Ar = Ur @ Vr.T
+            # the GPU drops rows of B carrying at most (1e-5 tol)^2 of its
+            # energy before the core (counted in the rank rule's tail)
+            bound = (1e-9 + 1e-4 * tol) * np.linalg.norm(D)
+            assert np.linalg.norm(Ag - Ar) <= bound
