@@ -207,6 +207,9 @@ int starsh_hip_fp64_probe(double *mfma_tflops, double *valu_tflops);
  *                         its sketch's singular values, exceeds this
  *                         factor times maxrank (default 1.5; 0 disables,
  *                         values in (0, 1) are refused);
+ *   "qr_graph"            Gram-Schmidt panels as one captured hipGraph,
+ *                         kept and replayed while its arguments are
+ *                         unchanged (1), or host launches (0, default);
  *   "tile_schedule"       per-tile schedules recorded by the autotuner
  *                         used by the mixed form (1, default) or not (0);
  *   "schedule_clear"      (value ignored) empties the autotuner's

@@ -60,6 +60,8 @@ int starsh_hip_set_option(const char *name, double value)
         return starsh_hip_gen_gemm_rw((int)value);
     if(!strcmp(name, "qr_panel"))
         return starsh_hip_qr_panel((int)value);
+    if(!strcmp(name, "qr_graph"))
+        return starsh_hip_qr_graph((int)value);
     if(!strcmp(name, "proj_int8"))
         return starsh_hip_proj_int8((int)value);
     if(!strcmp(name, "jacobi_max_sweeps"))

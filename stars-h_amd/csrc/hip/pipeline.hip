@@ -132,6 +132,40 @@ constexpr int PANEL8_TILES = 1536;
 // MFMA (0); set by starsh_hip_set_option("proj_int8").
 int proj_int8 = 1;
 
+// Gram-Schmidt panels as one captured hipGraph (option "qr_graph" 1)
+// instead of host launches pruned by the lagged probe: every panel up to
+// the batch's width is in the graph (finished tiles return at once), and
+// the executable graph is kept and replayed while its arguments (the
+// pooled buffers, sizes and thresholds) are unchanged, so a repeated
+// compression of one problem captures once.
+int qr_graph = 0;
+
+struct PanelGraph
+{
+    struct Key
+    {
+        const void *tiles, *ws, *keep, *active;
+        int ntiles, which, kmax, width_max, rows_max, pb;
+        double tau_first;
+        bool operator==(const Key &o) const
+        {
+            return tiles == o.tiles && ws == o.ws && keep == o.keep &&
+                active == o.active && ntiles == o.ntiles &&
+                which == o.which && kmax == o.kmax &&
+                width_max == o.width_max && rows_max == o.rows_max &&
+                pb == o.pb && tau_first == o.tau_first;
+        }
+    } key{};
+    hipGraphExec_t exec = nullptr;
+    void reset()
+    {
+        if(exec)
+            (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+    }
+};
+PanelGraph panel_graphs[2];      // which = 0, 1
+
 int bcgs2(TileDesc *d_tiles, int ntiles, int which, double *ws, int kmax,
         int width_max, int rows_max, int *d_keep, double tau_first,
         int *d_active, hipStream_t s)
@@ -139,6 +173,59 @@ int bcgs2(TileDesc *d_tiles, int ntiles, int which, double *ws, int kmax,
     const int pw = qr_panel_width ? qr_panel_width : qr_panel_tuned;
     const bool wide = pw == 8 || (pw == 0 && ntiles > PANEL8_TILES);
     const int pb = rows_max <= 2048 && wide ? 8 : 4;
+    if(qr_graph)
+    {
+        PanelGraph &pg = panel_graphs[which ? 1 : 0];
+        const PanelGraph::Key key{d_tiles, ws, d_keep, d_active, ntiles,
+            which, kmax, width_max, rows_max, pb, tau_first};
+        TRY(pipeline_count_active(d_tiles, ntiles, which, d_active, s));
+        // captured and launched on a stream of its own, ordered after and
+        // before the library stream's work by events: the copy-back threads
+        // queue transfers on the library stream, and those run at the
+        // library stream's rate only if no graph is launched on it
+        static hipStream_t cs = nullptr;
+        static hipEvent_t before = nullptr, after = nullptr;
+        if(cs == nullptr)
+        {
+            HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&before, hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&after, hipEventDisableTiming));
+        }
+        if(!(pg.exec && pg.key == key))
+        {
+            pg.reset();
+            hipGraph_t graph = nullptr;
+            HIP_TRY(hipStreamBeginCapture(cs,
+                        hipStreamCaptureModeThreadLocal));
+            int info = STARSH_SUCCESS;
+            for(int j0 = 0; info == STARSH_SUCCESS && j0 < width_max;
+                    j0 += pb)
+                info = pipeline_panel(d_tiles, ntiles, which, ws, j0, pb,
+                        kmax, tau_first, TAU_SECOND, d_keep, rows_max,
+                        d_active, cs);
+            hipError_t e = hipStreamEndCapture(cs, &graph);
+            if(info != STARSH_SUCCESS || e != hipSuccess)
+            {
+                if(graph)
+                    (void)hipGraphDestroy(graph);
+                return info != STARSH_SUCCESS ? info : STARSH_UNKNOWN_ERROR;
+            }
+            e = hipGraphInstantiate(&pg.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if(e != hipSuccess)
+            {
+                pg.exec = nullptr;
+                return STARSH_UNKNOWN_ERROR;
+            }
+            pg.key = key;
+        }
+        HIP_TRY(hipEventRecord(before, s));
+        HIP_TRY(hipStreamWaitEvent(cs, before, 0));
+        HIP_TRY(hipGraphLaunch(pg.exec, cs));
+        HIP_TRY(hipEventRecord(after, cs));
+        HIP_TRY(hipStreamWaitEvent(s, after, 0));
+        return STARSH_SUCCESS;
+    }
     TRY(probe.init());
     TRY(pipeline_count_active(d_tiles, ntiles, which, d_active, s));
     int panel = 0;
@@ -603,6 +690,8 @@ void rsdd_release_pools(void)
     for(Pool *q : {&tiles_pool, &keep_pool, &norms_pool, &om8_pool,
                 &active_pool, &qd_pool, &part_pool, &sel_pool, &piv_pool})
         q->release();
+    for(PanelGraph &pg : panel_graphs)
+        pg.reset();
     rsdd_out_release();
 }
 
@@ -619,6 +708,14 @@ extern "C" int hip_set_tuned_panel(int width)
     if(width != 0 && width != 4 && width != 8)
         return STARSH_WRONG_PARAMETER;
     qr_panel_tuned = width;
+    return STARSH_SUCCESS;
+}
+
+extern "C" int starsh_hip_qr_graph(int on)
+{
+    if(on != 0 && on != 1)
+        return STARSH_WRONG_PARAMETER;
+    qr_graph = on;
     return STARSH_SUCCESS;
 }
 

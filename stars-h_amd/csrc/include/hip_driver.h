@@ -141,6 +141,8 @@ int starsh_hip_gen_gemm_bn(int bn);
 int starsh_hip_gen_gemm_rw(int rw);
 /* Panel width of the block Gram-Schmidt (8 or 4). */
 int starsh_hip_qr_panel(int width);
+/* Gram-Schmidt panels as one captured hipGraph (1) or host launches (0). */
+int starsh_hip_qr_graph(int on);
 /* Values chosen by the autotuner for the current call (0 = by size); the
  * options above take precedence.  proj: 0 sketch GEMM, 1 projection. */
 int hip_set_tuned_gemm(int proj, int bn, int rw);
