@@ -161,8 +161,9 @@ def run_form(S, w, backend, steps, warmup, autotune, lrengine="RSVD"):
                         [round(x, 6) for x in v] if isinstance(v, list)
                         else v) for k, v in t.items()}
         if mixed:
-            schedule["first_bucket_width"] = [32, 64, 128, 256, 512][
-                t["first_bucket"]]
+            schedule["first_bucket_width"] = (
+                [32, 64, 128, 256, 384, 512] if t["ladder"] else
+                [32, 64, 128, 256, 512])[t["first_bucket"]]
     if mixed:
         # the timed steps do not replay the per-tile outcomes recorded on
         # this matrix by the plan step (reported apart below)

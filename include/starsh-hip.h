@@ -210,6 +210,13 @@ int starsh_hip_fp64_probe(double *mfma_tflops, double *valu_tflops);
  *   "qr_graph"            Gram-Schmidt panels as one captured hipGraph,
  *                         kept and replayed while its arguments are
  *                         unchanged (1), or host launches (0, default);
+ *   "mixed_ladder"        bucket ladder of the mixed form: 0 (widths 32,
+ *                         64, 128, 256, 512; the next bucket from the
+ *                         extrapolated rank), 1 (a 384 bucket added; the
+ *                         next bucket from the computed rank when the
+ *                         sketch was not saturated), or -1 (default) for
+ *                         the autotuner's schedule table (0 without an
+ *                         entry);
  *   "tile_schedule"       per-tile schedules recorded by the autotuner
  *                         used by the mixed form (1, default) or not (0);
  *   "schedule_clear"      (value ignored) empties the autotuner's
@@ -250,7 +257,8 @@ int starsh_hip_autotune(struct starsh_blrf *format, int maxrank, double tol,
 typedef struct starsh_hip_tuning
 {
     int form;               /* 0 plain FP64, 1 mixed */
-    int first_bucket;       /* mixed: chosen first bucket (0..4), else -1 */
+    int first_bucket;       /* mixed: chosen first bucket (0..4 on ladder
+                             * 0, 0..5 on ladder 1), else -1 */
     int tile_schedule;      /* mixed: per-tile schedule kept (1) or not */
     int qr_panel;           /* chosen Gram-Schmidt panel width (0: by batch,
                              * 4, 8) */
@@ -264,6 +272,9 @@ typedef struct starsh_hip_tuning
                                  * schedule, -1 if none was recorded */
     double untuned_seconds;     /* device time with the built-in choices */
     double best_seconds;        /* device time with the chosen knobs */
+    int ladder;                 /* mixed: chosen bucket ladder (0, 1) */
+    double ladder_seconds;      /* mixed: device time on ladder 1 at the
+                                 * chosen first width, -1 if not tried */
 } starsh_hip_tuning;
 
 /* Autotuner of both HIP forms for one problem or a batch (no reference
@@ -274,8 +285,13 @@ typedef struct starsh_hip_tuning
  *   plain: sketch GEMM (column block 256 or 128, one or two 16-row groups
  *          per wave), then projection GEMM (128 or 64, the same), then the
  *          Gram-Schmidt panel width (4, 8), each kept if faster;
- *   mixed: the first bucket as starsh_hip_autotune; then one compression
- *          at that bucket recording every far-field tile's outcome (the
+ *   mixed: the first bucket as starsh_hip_autotune; then the bucket
+ *          ladder: 0 (widths 32, 64, 128, 256, 512; a tile that did not
+ *          finish moves to the width its extrapolated rank needs) or 1
+ *          (a 384 bucket added; the rank a round computed steers the next
+ *          bucket when that round's sketch was not saturated); then one
+ *          compression at that bucket recording every far-field tile's
+ *          outcome (the
  *          bucket it finished in, or false far-field) and one with that
  *          per-tile schedule (every tile starts in its bucket, false
  *          far-field tiles are stored densely at once), kept if faster;

@@ -7,7 +7,8 @@
  *   plain form  the column block and row groups of the generated-tile
  *               sketch GEMM and of the projection GEMM, then the panel
  *               width of the block Gram-Schmidt;
- *   mixed form  the first rank bucket (sketch width 32..512), then a
+ *   mixed form  the first rank bucket (sketch width 32..512), then the
+ *               bucket ladder (with or without a 384-wide bucket), then a
  *               per-tile schedule (the bucket each far-field tile finished
  *               in, or "false far-field"), kept only if it is faster, then
  *               the Gram-Schmidt panel width.
@@ -22,15 +23,18 @@
  * Entries also go to the text file named by STARSH_HIP_SCHEDULE (read on
  * first use, appended to by the tuner), one line per entry:
  *   family ndim beta nu noise sigma wave_k diag n nb maxrank os tol form
- *   first panel sketch_bn sketch_rw proj_bn proj_rw geom ntile tiles
- * tiles: one character per far-field tile, '0'-'4' a bucket, 'd' false
- * far-field; "-" when there is no per-tile schedule. */
+ *   first ladder panel sketch_bn sketch_rw proj_bn proj_rw geom ntile tiles
+ * tiles: one character per far-field tile, '0'-'4' ('0'-'5' on ladder 1) a
+ * bucket of the entry's ladder, 'd' false far-field; "-" when there is no
+ * per-tile schedule. */
 #include <math.h>
 #include "common.h"
 #include "kernels_host.h"
 #include "rsdd.h"
 #include "hip_driver.h"
 
+/* Buckets of ladder 0, on which the first bucket is tuned (ladder 1 has
+ * one more, 384 wide, below the last). */
 #define NBUCKETS 5
 /* First bucket without a table entry: width 64.  On every workload
  * measured (2D sqexp and exp, 3D Matern, Coulomb and sin(kr)/r) width 64
@@ -66,7 +70,8 @@ typedef struct sched_entry
     int nb, maxrank, os;
     double tol;
     int form;               /* 0 plain, 1 mixed */
-    int first;              /* mixed: first bucket */
+    int first;              /* mixed: first bucket (of `ladder`) */
+    int ladder;             /* mixed: bucket ladder (0, 1) */
     int panel;              /* Gram-Schmidt panel width, 0 = by batch */
     int gemm_bn[2], gemm_rw[2];     /* [0] sketch, [1] projection */
     unsigned long long geom;
@@ -154,9 +159,10 @@ static void sched_put(sched_entry *e)
 
 static int valid_knobs(const sched_entry *e)
 {
-    int ok = (e->form == 0 || e->form == 1) && e->first >= -1 &&
-        e->first < NBUCKETS && (e->panel == 0 || e->panel == 4 ||
-                e->panel == 8);
+    int ok = (e->form == 0 || e->form == 1) &&
+        (e->ladder == 0 || e->ladder == 1) && e->first >= -1 &&
+        e->first < hip_mixed_nbuckets(e->ladder) && (e->panel == 0 ||
+                e->panel == 4 || e->panel == 8);
     for(int j = 0; j < 2; j++)
         ok = ok && (e->gemm_bn[j] == 0 || e->gemm_bn[j] == 64 ||
                 e->gemm_bn[j] == 128 || e->gemm_bn[j] == 256) &&
@@ -176,11 +182,11 @@ static void sched_load(void)
     sched_entry e;
     memset(&e, 0, sizeof(e));
     while(fscanf(f, "%d %d %lg %lg %lg %lg %lg %lg %lld %d %d %d %lg %d %d "
-                "%d %d %d %d %d %llu %d", &e.kd.family, &e.kd.ndim,
+                "%d %d %d %d %d %d %llu %d", &e.kd.family, &e.kd.ndim,
                 &e.kd.beta, &e.kd.nu, &e.kd.noise, &e.kd.sigma, &e.kd.wave_k,
                 &e.kd.diag, &e.n, &e.nb, &e.maxrank, &e.os, &e.tol, &e.form,
-                &e.first, &e.panel, &e.gemm_bn[0], &e.gemm_rw[0],
-                &e.gemm_bn[1], &e.gemm_rw[1], &e.geom, &e.ntile) == 22)
+                &e.first, &e.ladder, &e.panel, &e.gemm_bn[0], &e.gemm_rw[0],
+                &e.gemm_bn[1], &e.gemm_rw[1], &e.geom, &e.ntile) == 23)
     {
         e.tiles = NULL;
         int ok = valid_knobs(&e) && e.ntile >= 0 && e.ntile < (1 << 28);
@@ -196,7 +202,8 @@ static void sched_load(void)
             {
                 if(str[i] == 'd')
                     e.tiles[i] = -2;
-                else if(str[i] >= '0' && str[i] < '0'+NBUCKETS)
+                else if(str[i] >= '0' &&
+                        str[i] < '0'+hip_mixed_nbuckets(e.ladder))
                     e.tiles[i] = (signed char)(str[i]-'0');
                 else
                     ok = 0;
@@ -226,11 +233,12 @@ static void sched_save(const sched_entry *e)
     if(f == NULL)
         return;
     fprintf(f, "%d %d %.17g %.17g %.17g %.17g %.17g %.17g %lld %d %d %d "
-            "%.17g %d %d %d %d %d %d %d %llu %d ", e->kd.family, e->kd.ndim,
-            e->kd.beta, e->kd.nu, e->kd.noise, e->kd.sigma, e->kd.wave_k,
-            e->kd.diag, e->n, e->nb, e->maxrank, e->os, e->tol, e->form,
-            e->first, e->panel, e->gemm_bn[0], e->gemm_rw[0], e->gemm_bn[1],
-            e->gemm_rw[1], e->geom, e->tiles ? e->ntile : 0);
+            "%.17g %d %d %d %d %d %d %d %d %llu %d ", e->kd.family,
+            e->kd.ndim, e->kd.beta, e->kd.nu, e->kd.noise, e->kd.sigma,
+            e->kd.wave_k, e->kd.diag, e->n, e->nb, e->maxrank, e->os, e->tol,
+            e->form, e->first, e->ladder, e->panel, e->gemm_bn[0],
+            e->gemm_rw[0], e->gemm_bn[1], e->gemm_rw[1], e->geom,
+            e->tiles ? e->ntile : 0);
     if(e->tiles && e->ntile > 0)
         for(int i = 0; i < e->ntile; i++)
             fputc(e->tiles[i] == -2 ? 'd' : '0'+e->tiles[i], f);
@@ -255,15 +263,16 @@ static const sched_entry *sched_find(const sched_entry *key)
  * between buckets and stores tiles densely (exactly), so one recorded for
  * other parameters can cost time, never accuracy.  NULL when none applies
  * to ntile far-field tiles. */
-static const signed char *sched_tiles(const sched_entry *key, STARSH_int ntile)
+static const sched_entry *sched_tiles(const sched_entry *key,
+        STARSH_int ntile)
 {
     const sched_entry *e = sched_find(key);
     if(e && e->tiles && e->ntile == ntile)
-        return e->tiles;
+        return e;
     for(int i = nsched-1; i >= 0; i--)
         if(sched[i].tiles && sched[i].ntile == ntile &&
                 sched_same_geom(&sched[i], key))
-            return sched[i].tiles;
+            return &sched[i];
     return NULL;
 }
 
@@ -301,9 +310,10 @@ void hip_sched_apply(int mixed, const hip_batch *B)
     hip_set_tuned_panel(e ? e->panel : 0);
     for(int j = 0; j < 2; j++)
         hip_set_tuned_gemm(j, e ? e->gemm_bn[j] : 0, e ? e->gemm_rw[j] : 0);
+    hip_mixed_tuned_ladder(mixed && e ? e->ladder : 0, trial_on);
 }
 
-void hip_sched_plan(const hip_batch *B, int q, int *first,
+void hip_sched_plan(const hip_batch *B, int q, int ladder, int *first,
         const signed char **tiles)
 {
     *tiles = NULL;
@@ -322,8 +332,12 @@ void hip_sched_plan(const hip_batch *B, int q, int *first,
     sched_key(1, B->hp+q, B->F[q], B->maxrank[q], B->tol[q], &key);
     const sched_entry *e = sched_find(&key);
     *first = e && e->first >= 0 ? e->first : DEFAULT_FIRST;
-    if(use_tile_schedule)
-        *tiles = sched_tiles(&key, B->F[q]->nblocks_far);
+    /* a per-tile schedule names buckets of its own ladder: used on that
+     * ladder only */
+    const sched_entry *s = use_tile_schedule ?
+        sched_tiles(&key, B->F[q]->nblocks_far) : NULL;
+    if(s && s->ladder == ladder)
+        *tiles = s->tiles;
 }
 
 void hip_sched_record(int q, STARSH_int tile, int outcome)
@@ -519,6 +533,25 @@ static int tune_mixed(tuner *T, starsh_hip_tuning *out, signed char ***rec)
             return info;
     }
     trial.first = out->first_bucket;
+    /* ladder 1 (a 384-wide bucket, next bucket from the computed rank):
+     * the same first width (the last bucket moves up one index) */
+    trial.ladder = 1;
+    if(trial.first == NBUCKETS-1)
+        trial.first = NBUCKETS;
+    info = tune_run(T, &out->ladder_seconds);
+    if(info != STARSH_SUCCESS)
+        return info;
+    if(out->ladder_seconds < best_t)
+    {
+        best_t = out->ladder_seconds;
+        out->ladder = 1;
+        out->first_bucket = trial.first;
+    }
+    else
+    {
+        trial.ladder = 0;
+        trial.first = out->first_bucket;
+    }
     /* record every tile's outcome at the chosen first bucket */
     signed char **r = calloc(T->nprob, sizeof(*r));
     if(r == NULL)
@@ -599,6 +632,7 @@ int starsh_hip_autotune_batch(STARSH_blrf **format, int nprob,
     out->first_bucket = -1;
     out->schedule_seconds = -1.0;
     out->untuned_seconds = -1.0;
+    out->ladder_seconds = -1.0;
     for(int b = 0; b < NBUCKETS; b++)
         out->first_seconds[b] = -1.0;
     tuner T = {format, nprob, form, onfly, maxrank, tol, NULL, 0};
@@ -631,6 +665,7 @@ int starsh_hip_autotune_batch(STARSH_blrf **format, int nprob,
         sched_key(form, &hp, format[q], maxrank[q], tol[q], e+q);
         hip_driver_release(&hp);
         e[q].first = form ? out->first_bucket : -1;
+        e[q].ladder = form ? out->ladder : 0;
         e[q].panel = trial.panel;
         for(int j = 0; j < 2; j++)
         {

@@ -70,6 +70,8 @@ int starsh_hip_set_option(const char *name, double value)
         return hip_mixed_set_eta(value);
     if(!strcmp(name, "mixed_early_dense"))
         return hip_mixed_set_early_dense(value);
+    if(!strcmp(name, "mixed_ladder"))
+        return hip_mixed_set_ladder((int)value);
     if(!strcmp(name, "tile_schedule"))
         return hip_mixed_set_tile_schedule((int)value);
     if(!strcmp(name, "schedule_clear"))

@@ -21,8 +21,45 @@
 #include "rsdd.h"
 #include "hip_driver.h"
 
-static const int bucket_widths[] = {32, 64, 128, 256, 512};
-#define NBUCKETS ((int)(sizeof(bucket_widths)/sizeof(bucket_widths[0])))
+/* Sketch widths of the rank buckets (the ladder).  Ladder 0: 32, 64, 128,
+ * 256, 512.  Ladder 1 adds 384, and a tile that did not finish moves to
+ * the bucket that fits the rank its round computed when that round's
+ * sketch was not saturated (rank + 2 oversample <= width), instead of the
+ * rank extrapolated from the decay of its singular values, which
+ * overestimates the ranks of slowly decaying tiles (3D Matern: tiles of
+ * rank 145-192 extrapolate to 222-833 from a 256-wide round).  Ladder 1
+ * takes 5% off the 3D Matern workload and adds 15-19% to others (more,
+ * smaller batches per round), so the autotuner chooses it per problem.
+ * The last bucket is always the plain width.  The ladder of a call: the
+ * option "mixed_ladder" when set (0/1), else the autotuner's. */
+static const int ladders[2][6] = {
+    {32, 64, 128, 256, 512},
+    {32, 64, 128, 256, 384, 512}};
+static const int ladder_len[2] = {5, 6};
+static int ladder_forced = -1, ladder_tuned = 0, ladder_trial = 0;
+static int ladder = 0;                  /* of the current call */
+#define bucket_widths (ladders[ladder])
+#define NBUCKETS (ladder_len[ladder])
+#define NBUCKETS_MAX 6
+
+int hip_mixed_set_ladder(int l)
+{
+    if(l < -1 || l > 1)
+        return STARSH_WRONG_PARAMETER;
+    ladder_forced = l;
+    return STARSH_SUCCESS;
+}
+
+void hip_mixed_tuned_ladder(int l, int trial)
+{
+    ladder_tuned = l == 1 ? 1 : 0;
+    ladder_trial = trial;
+}
+
+int hip_mixed_nbuckets(int l)
+{
+    return ladder_len[l == 1 ? 1 : 0];
+}
 
 /* Early false far-field decision: a tile that has not finished and whose
  * rank, extrapolated from the singular values of its sketch, exceeds
@@ -112,6 +149,8 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
 {
     if(ntiles == 0)
         return STARSH_SUCCESS;
+    ladder = ladder_forced >= 0 && !ladder_trial ? ladder_forced :
+        ladder_tuned;
     int info = STARSH_SUCCESS;
     /* pending tiles (positions into pidx/idx) and their current bucket */
     STARSH_int *pend = malloc(sizeof(STARSH_int)*ntiles);
@@ -160,12 +199,18 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
     if(tsched == NULL)
         info = STARSH_MALLOC_ERROR;
     for(int q = 0; info == STARSH_SUCCESS && q < B->nprob; q++)
-        hip_sched_plan(B, q, first+q, tsched+q);
+    {
+        hip_sched_plan(B, q, ladder, first+q, tsched+q);
+        if(first[q] < 0 || first[q] >= NBUCKETS)
+            first[q] = NBUCKETS-1;
+    }
     STARSH_int npend = 0;
     for(STARSH_int t = 0; info == STARSH_SUCCESS && t < ntiles; t++)
     {
         const int q = pidx[t];
-        const int b = tsched[q] ? tsched[q][idx[t]] : first[q];
+        int b = tsched[q] ? tsched[q][idx[t]] : first[q];
+        if(b != -2 && (b < 0 || b >= NBUCKETS))
+            b = NBUCKETS-1;
         if(b == -2)
         {
             /* a false far-field tile in the tuning run: dense at once */
@@ -184,7 +229,7 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
         st->tile_rounds += npend;
         /* pending tiles grouped by bucket (stable), one batch per group so
          * that every batch has one sketch width class */
-        STARSH_int cnt[NBUCKETS+1];
+        STARSH_int cnt[NBUCKETS_MAX+1];
         memset(cnt, 0, sizeof(cnt));
         for(STARSH_int u = 0; u < npend; u++)
             cnt[bucket[pend[u]]+1]++;
@@ -273,7 +318,9 @@ int hip_mixed_tiles(const hip_batch *B, STARSH_int ntiles, const int *pidx,
                     hip_sched_record(q, pi[u], -2);
                     continue;
                 }
-                if(r < rk[u])
+                if(ladder == 1 && rk[u] >= 0 && rk[u]+2*os <= k[u])
+                    r = rk[u];
+                else if(r < rk[u])
                     r = rk[u];
                 const double want = r+(r > os ? r : os);
                 int b = bucket[t]+1;

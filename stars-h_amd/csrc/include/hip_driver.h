@@ -151,11 +151,17 @@ int hip_set_tuned_panel(int width);
  * trial knobs while tuning) before a compression; mixed: 0 plain, 1 mixed
  * form. */
 void hip_sched_apply(int mixed, const hip_batch *B);
+/* Bucket ladder of the mixed form: option (-1 = the autotuner's, 0, 1),
+ * the autotuner's choice for the current call (a tuning trial's wins over
+ * the option), and the bucket count of a ladder. */
+int hip_mixed_set_ladder(int l);
+void hip_mixed_tuned_ladder(int l, int trial);
+int hip_mixed_nbuckets(int l);
 /* Per-tile schedules of the mixed form on (1, default) or off (0). */
 int hip_mixed_set_tile_schedule(int on);
 /* First bucket and per-tile schedule (or NULL) of problem q of a mixed
- * compression. */
-void hip_sched_plan(const hip_batch *B, int q, int *first,
+ * compression on bucket ladder `ladder`. */
+void hip_sched_plan(const hip_batch *B, int q, int ladder, int *first,
         const signed char **tiles);
 /* Outcome of far-field tile `tile` of problem q (final bucket, or -2 for a
  * false far-field tile), kept while the autotuner records. */

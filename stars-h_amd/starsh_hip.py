@@ -245,7 +245,9 @@ class Tuning(C.Structure):
                 ("first_seconds", C.c_double * 5),
                 ("schedule_seconds", C.c_double),
                 ("untuned_seconds", C.c_double),
-                ("best_seconds", C.c_double)]
+                ("best_seconds", C.c_double),
+                ("ladder", C.c_int),
+                ("ladder_seconds", C.c_double)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}

@@ -46,6 +46,7 @@ def _clean_table():
     _opt("schedule_clear", 0)
     _opt("tile_schedule", 1)
     _opt("mixed_first_bucket", -1)
+    _opt("mixed_ladder", -1)
 
 
 def _problem(beta, seed=7, N=20000, nb=1000):
@@ -87,16 +88,22 @@ def test_mixed_tuning_records_and_replays_tile_schedule():
          "maxrank 120")
     t = S.autotune_batch([make()], maxrank, tol, form=1)
     _say(f"  tuning {t}")
-    assert 0 <= t["first_bucket"] <= 4
+    assert t["ladder"] in (0, 1)
+    assert 0 <= t["first_bucket"] <= 4 + t["ladder"]
     assert t["qr_panel"] in (0, 4, 8)
     assert t["untuned_seconds"] > 0 and t["schedule_seconds"] > 0
+    assert t["ladder_seconds"] > 0
     assert t["best_seconds"] <= t["untuned_seconds"]
-    assert t["first_seconds"][t["first_bucket"]] > 0
-    assert t["ntrials"] >= 4
-    # the recorded run: chosen first bucket, no per-tile schedule
+    assert t["best_seconds"] <= t["ladder_seconds"]
+    if t["ladder"] == 0:
+        assert t["first_seconds"][t["first_bucket"]] > 0
+    assert t["ntrials"] >= 5
+    # the recorded run: chosen first bucket and ladder, no per-tile schedule
     _opt("mixed_first_bucket", t["first_bucket"])
+    _opt("mixed_ladder", t["ladder"])
     tiles0, far0, st0 = _run(make, fn, maxrank, tol)
     _opt("mixed_first_bucket", -1)
+    _opt("mixed_ladder", -1)
     ndense = len(tiles0) - len(far0)
     assert ndense > 0 and st0.nfalse_far == ndense
     # through the table: the per-tile schedule when the tuner kept it
@@ -169,7 +176,7 @@ def test_batch_tuning():
     _say("case: batch autotune, 4 2D exp problems")
     t = S.autotune_batch(make(), maxrank, tol, form=1)
     _say(f"  tuning {t}")
-    assert 0 <= t["first_bucket"] <= 4
+    assert 0 <= t["first_bucket"] <= 4 + t["ladder"]
     ps = make()
     nfar = sum(len(p.tiles()) for p in ps)
     assert S.approximate_batch(ps, maxrank, tol) == 0
@@ -234,8 +241,9 @@ def _child(mode, path):
 
 def test_schedule_file_round_trip(tmp_path):
     """The tuner appends its entry to STARSH_HIP_SCHEDULE; the entry with
-    its per-tile schedule replaced by a hand-made one (every tile in the
-    widest bucket, every 7th stored densely) is read by a new process, which
+    its per-tile schedule replaced by a hand-made one (ladder 0, every tile
+    in the widest bucket, every 7th stored densely) is read by a new process,
+    which
     must follow it: the marked tiles move to the near field, the others get
     the ranks of a compression that starts every tile in the widest bucket,
     each tile in one round."""
@@ -247,15 +255,17 @@ def test_schedule_file_round_trip(tmp_path):
     entry = [x for x in path.read_text().split("\n") if x.strip()]
     assert len(entry) == 1
     fields = entry[0].split()
-    assert len(fields) == 23
+    assert len(fields) == 24
     assert int(fields[13]) == 1 and int(fields[14]) == t["first_bucket"]
+    assert int(fields[15]) == t["ladder"]
     if t["tile_schedule"]:
-        assert int(fields[21]) == len(fields[22]) == nfar
-        assert set(fields[22]) <= set("01234d")
+        assert int(fields[22]) == len(fields[23]) == nfar
+        assert set(fields[23]) <= set("012345d"[:5 + t["ladder"]] + "d")
     else:
-        assert fields[21] == "0" and fields[22] == "-"
+        assert fields[22] == "0" and fields[23] == "-"
     sched = "".join("d" if i % 7 == 3 else "4" for i in range(nfar))
-    fields[21], fields[22] = str(nfar), sched
+    fields[15] = "0"
+    fields[22], fields[23] = str(nfar), sched
     path.write_text(" ".join(fields) + "\n")
     _say("case: schedule file, replaying process")
     r = _child("replay", path)
