@@ -164,12 +164,14 @@ void starsh_hip_tlr_free(starsh_hip_tlr *T);
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
  * It computes the same product with the same kernels and allocates nothing:
- * the r x nrhs coefficient blocks of the far-field tiles, which
- * starsh_hip_tlr_dmml keeps in a workspace of T grown on demand, go to a
- * static device array of 2^22 doubles (32 MiB) that each GPU holds until
- * the process ends.  Products whose coefficients need more than that,
- * (2 x sum of far-field ranks) x nrhs doubles, return
- * STARSH_WRONG_PARAMETER.  All calls on one GPU share the array: calls that
+ * the r x nrhs coefficient blocks of the far-field tiles and the n x nrhs
+ * transposed contributions of the symmetric format's off-diagonal tiles,
+ * which starsh_hip_tlr_dmml keeps in a workspace of T grown on demand, go
+ * to a static device array of 2^22 doubles (32 MiB) that each GPU holds
+ * until the process ends.  Products that need more than that,
+ * (2 x sum of far-field ranks + sum of n over those tiles) x nrhs doubles,
+ * return STARSH_WRONG_PARAMETER (2D sqexp N = 64000, tile 1600: up to 3
+ * right-hand sides).  All calls on one GPU share the array: calls that
  * overlap on one GPU (from several host threads) race. */
 int starsh_hip_tlr_dmml_noalloc(starsh_hip_tlr *T, int nrhs, double alpha,
         const double *d_A, int lda, double beta, double *d_B, int ldb);

@@ -6,13 +6,18 @@
 // tiles add D X_j (and D^T X_i).
 //
 // Two launches, both deterministic (no atomics):
-//  1. tlr_coef_kernel: per far-field tile and direction, the r x nrhs
-//     coefficients T = V^T X_j (and T' = U^T X_i); one workgroup per tile
-//     and direction, wave w takes factor columns w, w+4, ...;
+//  1. tlr_coef_kernel, one workgroup per far-field tile: the r x nrhs
+//     coefficients T = V^T X_j; for a tile with a twin (symmetric format,
+//     i != j) it first forms T' = U^T X_i (one read of U), then takes T and
+//     the twin's whole contribution P = V T' (n x nrhs) from one read of V;
 //  2. tlr_rows_kernel: per chunk of 64 rows of one block row, the sum of all
-//     contributions to those rows (U T, V T', D X_j, D^T X_i) in a fixed
+//     contributions to those rows (U T, P, D X_j, D^T X_i) in a fixed
 //     order: lane = row, wave w takes entries w, w+4, ... of the block row,
 //     and the four partial sums are added in LDS in wave order.
+// A twin tile's U is read twice (for T' and for U T) and its V once, 1.5
+// reads per factor instead of 2 for coefficient and row passes that each
+// read both factors: a factor of a 1600-row tile does not stay on a CU
+// between two uses.
 #include <stdio.h>
 #include "hip_common.h"
 #include "tlr.h"
@@ -20,52 +25,130 @@
 namespace {
 
 constexpr int RHS_GROUP = 4;
+constexpr int COL_BLOCK = 16;
 
-__global__ __launch_bounds__(256) void tlr_coef_kernel(const tlr_tile *tiles,
-        const double *fac, const double *x, int ldx, int nrhs, double *tmp)
+// One pass over a factor F (len x r, ld len) for right-hand sides q0 ..
+// q0+nq-1 (nq <= NQ, NQ = 1 or RHS_GROUP): column sums
+// S[c][q] = sum_l F[l][c] X[l][q] into S + c + q r, and with Tp (r x nrhs,
+// ld r) also the rows P[l][q] = sum_c F[l][c] Tp[c][q] into P + l + q len.
+// Columns go in blocks of 16 (8) with 64-row chunks of the factor striped over
+// the four waves (lane = row); a block's column sums are reduced over the
+// lanes by shuffles and over the waves in LDS in wave order, and each row
+// of P is owned by one lane, which adds the blocks in column order.
+template <int NQ>
+__device__ __forceinline__ void factor_pass(const double *F, int len, int r,
+        const double *X, int ldx, int q0, int nq, double *S, const double *Tp,
+        double *P, double (*red)[COL_BLOCK][RHS_GROUP], double *tp)
 {
-    const tlr_tile t = tiles[blockIdx.x];
-    const int dir = blockIdx.y;
-    if(dir == 1 && !t.twin)
-        return;
+    // columns per block: 16, or 8 with four right-hand sides (registers)
+    constexpr int CB = NQ == 1 ? COL_BLOCK : COL_BLOCK/2;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // dir 0: T = V^T X[c0..c0+n); dir 1: T' = U^T X[r0..r0+m)
-    const double *F = fac+(dir == 0 ? t.v_off : t.u_off);
-    const int len = dir == 0 ? t.n : t.m;
-    const int64_t x0 = dir == 0 ? t.c0 : t.r0;
-    double *T = tmp+(t.tmp_off+(int64_t)dir*t.rank)*nrhs;
-    // columns wave + 4 (u + 4 g), u = 0..3, four at a time (independent
-    // sums, one load of X for all four)
-    for(int c0 = wave; c0 < t.rank; c0 += 16)
+    for(int cb = 0; cb < r; cb += CB)
     {
-        for(int q = 0; q < nrhs; q++)
+        const int nc = r-cb < CB ? r-cb : CB;
+        if(Tp && threadIdx.x < COL_BLOCK*RHS_GROUP)
         {
-            const double *xv = x+(int64_t)q*ldx+x0;
-            double s[4] = {0.0, 0.0, 0.0, 0.0};
-            // four rows per lane in flight at a time
-#pragma unroll 4
-            for(int l = lane; l < len; l += 64)
-            {
-                const double xl = xv[l];
-#pragma unroll
-                for(int u = 0; u < 4; u++)
-                    if(c0+4*u < t.rank)
-                        s[u] = fma(F[(int64_t)(c0+4*u)*len+l], xl, s[u]);
-            }
-#pragma unroll
-            for(int o = 32; o > 0; o >>= 1)
-#pragma unroll
-                for(int u = 0; u < 4; u++)
-                    s[u] += __shfl_xor(s[u], o, 64);
-            if(lane == 0)
-#pragma unroll
-                for(int u = 0; u < 4; u++)
-                    if(c0+4*u < t.rank)
-                        T[c0+4*u+(int64_t)q*t.rank] = s[u];
+            const int u = threadIdx.x % COL_BLOCK, q = threadIdx.x/COL_BLOCK;
+            tp[threadIdx.x] = u < nc && q < nq ?
+                Tp[cb+u+(int64_t)(q0+q)*r] : 0.0;
         }
+        __syncthreads();
+        double s[CB][NQ] = {};
+        // one chunk at a time (two in flight measured no faster)
+#pragma unroll 1
+        for(int l = wave*64+lane; l-lane < len; l += 256)
+        {
+            const bool ok = l < len;
+            // loads of the columns that exist only (unconditional loads
+            // from clamped addresses measured 35% slower on 1600-row tiles
+            // of rank 7-31: the extra loads cost more than the branches)
+            double x[NQ], v[CB];
+#pragma unroll
+            for(int q = 0; q < NQ; q++)
+                x[q] = ok && q < nq ? X[l+(int64_t)(q0+q)*ldx] : 0.0;
+#pragma unroll
+            for(int u = 0; u < CB; u++)
+                v[u] = ok && u < nc ? F[(int64_t)(cb+u)*len+l] : 0.0;
+#pragma unroll
+            for(int u = 0; u < CB; u++)
+#pragma unroll
+                for(int q = 0; q < NQ; q++)
+                    s[u][q] = fma(v[u], x[q], s[u][q]);
+            if(Tp && ok)
+#pragma unroll
+                for(int q = 0; q < NQ; q++)
+                    if(q < nq)
+                    {
+                        double p = 0.0;
+#pragma unroll
+                        for(int u = 0; u < CB; u++)
+                            p = fma(v[u], tp[u+q*COL_BLOCK], p);
+                        double *pp = P+l+(int64_t)q*len;
+                        *pp = cb == 0 ? p : *pp+p;
+                    }
+        }
+#pragma unroll
+        for(int u = 0; u < CB; u++)
+#pragma unroll
+            for(int q = 0; q < NQ; q++)
+                if(u < nc && q < nq)
+                {
+                    double t = s[u][q];
+#pragma unroll
+                    for(int o = 32; o > 0; o >>= 1)
+                        t += __shfl_xor(t, o, 64);
+                    s[u][q] = t;
+                }
+        if(lane == 0)
+#pragma unroll
+            for(int u = 0; u < CB; u++)
+#pragma unroll
+                for(int q = 0; q < NQ; q++)
+                    red[wave][u][q] = s[u][q];
+        __syncthreads();
+        if(threadIdx.x < COL_BLOCK*RHS_GROUP)
+        {
+            const int u = threadIdx.x % COL_BLOCK, q = threadIdx.x/COL_BLOCK;
+            if(u < nc && q < nq)
+                S[cb+u+(int64_t)(q0+q)*r] = ((red[0][u][q]+red[1][u][q])+
+                        red[2][u][q])+red[3][u][q];
+        }
+        __syncthreads();
     }
 }
 
+template <int NQ>
+__global__ __launch_bounds__(256) void tlr_coef_kernel(
+        const tlr_tile *tiles, const double *fac, const double *x, int ldx,
+        int nrhs, double *tmp)
+{
+    __shared__ double red[4][COL_BLOCK][RHS_GROUP];
+    __shared__ double tp[COL_BLOCK*RHS_GROUP];
+    const tlr_tile t = tiles[blockIdx.x];
+    if(t.rank <= 0)
+        return;
+    double *T = tmp+t.tmp_off*nrhs;            // V^T X_j, r x nrhs
+    double *Tt = tmp+(t.tmp_off+t.rank)*nrhs;  // U^T X_i
+    double *P = tmp+t.p_off*nrhs;              // V T', n x nrhs
+    for(int q0 = 0; q0 < nrhs; q0 += NQ)
+    {
+        const int nq = nrhs-q0 < NQ ? nrhs-q0 : NQ;
+        if(t.twin)
+        {
+            factor_pass<NQ>(fac+t.u_off, t.m, t.rank, x+t.r0, ldx, q0, nq,
+                    Tt, nullptr, nullptr, red, tp);
+            // Tt, written by other threads of this workgroup, is visible
+            // after the barrier that ended the pass
+            factor_pass<NQ>(fac+t.v_off, t.n, t.rank, x+t.c0, ldx, q0, nq,
+                    T, Tt, P+(int64_t)q0*t.n, red, tp);
+        }
+        else
+            factor_pass<NQ>(fac+t.v_off, t.n, t.rank, x+t.c0, ldx, q0, nq,
+                    T, nullptr, nullptr, red, tp);
+    }
+}
+
+template <int NQ>
 __global__ __launch_bounds__(256) void tlr_rows_kernel(const tlr_chunk *chunks,
         const tlr_entry *entries, const double *fac, const double *tmp,
         const double *x, int ldx, int nrhs, double alpha, double beta,
@@ -73,12 +156,15 @@ __global__ __launch_bounds__(256) void tlr_rows_kernel(const tlr_chunk *chunks,
 {
     __shared__ double part[4][RHS_GROUP][64];
     const tlr_chunk ch = chunks[blockIdx.x];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wave index as a wave-uniform value: entries and coefficients
+    // are then read through the scalar unit
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const bool row_ok = lane < ch.nrows;
-    for(int q0 = 0; q0 < nrhs; q0 += RHS_GROUP)
+    for(int q0 = 0; q0 < nrhs; q0 += NQ)
     {
-        const int nq = nrhs-q0 < RHS_GROUP ? nrhs-q0 : RHS_GROUP;
-        double acc[RHS_GROUP] = {0.0, 0.0, 0.0, 0.0};
+        const int nq = nrhs-q0 < NQ ? nrhs-q0 : NQ;
+        double acc[NQ] = {};
         for(int e = ch.first+wave; e < ch.last; e += 4)
         {
             const tlr_entry en = entries[e];
@@ -89,9 +175,24 @@ __global__ __launch_bounds__(256) void tlr_rows_kernel(const tlr_chunk *chunks,
                 // four independent partial sums over the factor columns
                 const double *F = fac+en.f_off+ch.rofs+lane;
                 const double *T = tmp+en.src_off*nrhs;
-                double a4[4][RHS_GROUP] = {};
+                double a4[4][NQ] = {};
                 int c = 0;
-                // eight factor columns in flight at a time
+                // sixteen, then eight factor columns in flight at a time
+                // (column c always goes to partial sum c mod 4)
+                for(; c+16 <= en.cols; c += 16)
+                {
+                    double f16[16];
+#pragma unroll
+                    for(int u = 0; u < 16; u++)
+                        f16[u] = F[(int64_t)(c+u)*en.ld];
+#pragma unroll
+                    for(int u = 0; u < 16; u++)
+#pragma unroll
+                        for(int q = 0; q < NQ; q++)
+                            if(NQ == 1 || q < nq)
+                                a4[u & 3][q] = fma(f16[u], T[c+u+
+                                        (int64_t)(q0+q)*en.cols], a4[u & 3][q]);
+                }
                 for(; c+8 <= en.cols; c += 8)
                 {
                     double f8[8];
@@ -101,8 +202,8 @@ __global__ __launch_bounds__(256) void tlr_rows_kernel(const tlr_chunk *chunks,
 #pragma unroll
                     for(int u = 0; u < 8; u++)
 #pragma unroll
-                        for(int q = 0; q < RHS_GROUP; q++)
-                            if(q < nq)
+                        for(int q = 0; q < NQ; q++)
+                            if(NQ == 1 || q < nq)
                                 a4[u & 3][q] = fma(f8[u], T[c+u+
                                         (int64_t)(q0+q)*en.cols], a4[u & 3][q]);
                 }
@@ -112,8 +213,8 @@ __global__ __launch_bounds__(256) void tlr_rows_kernel(const tlr_chunk *chunks,
                     {
                         const double f = F[(int64_t)(c+u)*en.ld];
 #pragma unroll
-                        for(int q = 0; q < RHS_GROUP; q++)
-                            if(q < nq)
+                        for(int q = 0; q < NQ; q++)
+                            if(NQ == 1 || q < nq)
                                 a4[u][q] = fma(f, T[c+u+(int64_t)(q0+q)*
                                         en.cols], a4[u][q]);
                     }
@@ -121,14 +222,23 @@ __global__ __launch_bounds__(256) void tlr_rows_kernel(const tlr_chunk *chunks,
                 {
                     const double f = F[(int64_t)c*en.ld];
 #pragma unroll
-                    for(int q = 0; q < RHS_GROUP; q++)
-                        if(q < nq)
+                    for(int q = 0; q < NQ; q++)
+                        if(NQ == 1 || q < nq)
                             a4[0][q] = fma(f, T[c+(int64_t)(q0+q)*en.cols],
                                     a4[0][q]);
                 }
 #pragma unroll
-                for(int q = 0; q < RHS_GROUP; q++)
+                for(int q = 0; q < NQ; q++)
                     acc[q] += (a4[0][q]+a4[1][q])+(a4[2][q]+a4[3][q]);
+            }
+            else if(en.kind == 3)
+            {
+                // a twin's contribution V T', formed by the coefficient pass
+                const double *P = tmp+en.src_off*nrhs+ch.rofs+lane;
+#pragma unroll
+                for(int q = 0; q < NQ; q++)
+                    if(NQ == 1 || q < nq)
+                        acc[q] += P[(int64_t)(q0+q)*en.ld];
             }
             else
             {
@@ -139,15 +249,15 @@ __global__ __launch_bounds__(256) void tlr_rows_kernel(const tlr_chunk *chunks,
                     const double f = en.kind == 1 ?
                         D[r+(int64_t)c*en.ld] : D[c+(int64_t)r*en.ld];
 #pragma unroll
-                    for(int q = 0; q < RHS_GROUP; q++)
-                        if(q < nq)
+                    for(int q = 0; q < NQ; q++)
+                        if(NQ == 1 || q < nq)
                             acc[q] = fma(f, x[en.src_off+c+
                                     (int64_t)(q0+q)*ldx], acc[q]);
                 }
             }
         }
 #pragma unroll
-        for(int q = 0; q < RHS_GROUP; q++)
+        for(int q = 0; q < NQ; q++)
             part[wave][q][lane] = acc[q];
         __syncthreads();
         if(wave == 0 && row_ok)
@@ -282,14 +392,24 @@ int starsh_hip_tlr_apply(const tlr_tile *d_tiles, int ntiles,
     hipStream_t s = starsh_hip_stream();
     if(ntiles > 0)
     {
-        hipLaunchKernelGGL(tlr_coef_kernel, dim3(ntiles, 2), dim3(256), 0,
-                s, d_tiles, d_fac, d_x, ldx, nrhs, d_tmp);
+        if(nrhs == 1)
+            hipLaunchKernelGGL(tlr_coef_kernel<1>, dim3(ntiles), dim3(256),
+                    0, s, d_tiles, d_fac, d_x, ldx, nrhs, d_tmp);
+        else
+            hipLaunchKernelGGL(tlr_coef_kernel<RHS_GROUP>, dim3(ntiles),
+                    dim3(256), 0, s, d_tiles, d_fac, d_x, ldx, nrhs, d_tmp);
         HIP_TRY(hipGetLastError());
     }
     if(nchunks > 0)
     {
-        hipLaunchKernelGGL(tlr_rows_kernel, dim3(nchunks), dim3(256), 0, s,
-                d_chunks, d_entries, d_fac, d_tmp, d_x, ldx, nrhs, alpha, beta, d_y, ldy);
+        if(nrhs == 1)
+            hipLaunchKernelGGL(tlr_rows_kernel<1>, dim3(nchunks), dim3(256),
+                    0, s, d_chunks, d_entries, d_fac, d_tmp, d_x, ldx, nrhs,
+                    alpha, beta, d_y, ldy);
+        else
+            hipLaunchKernelGGL(tlr_rows_kernel<RHS_GROUP>, dim3(nchunks),
+                    dim3(256), 0, s, d_chunks, d_entries, d_fac, d_tmp, d_x,
+                    ldx, nrhs, alpha, beta, d_y, ldy);
         HIP_TRY(hipGetLastError());
     }
     return STARSH_SUCCESS;

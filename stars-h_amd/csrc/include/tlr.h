@@ -11,20 +11,22 @@ extern "C" {
 
 /* One far-field tile: factors U (m x r, ld m) and V (n x r, ld n) at double
  * offsets of the factor buffer, its rows r0.. and columns c0.. of the
- * vectors, and its coefficients: T (r x nrhs, ld r) at tmp_off*nrhs of the
- * coefficient buffer and T' at (tmp_off + r)*nrhs, tmp_off counting 2r per
- * tile.  twin: the transposed product V (U^T X_i) is added as well
- * (symmetric format, i != j). */
+ * vectors, and its coefficients: T = V^T X_j (r x nrhs, ld r) at
+ * tmp_off*nrhs of the coefficient buffer and T' = U^T X_i at
+ * (tmp_off + r)*nrhs.  twin: the transposed product P = V T' is added as
+ * well (symmetric format, i != j); P (n x nrhs, ld n) is at p_off*nrhs.
+ * Offsets count 2r units per tile plus n per twin tile. */
 typedef struct
 {
-    int64_t u_off, v_off, tmp_off, r0, c0;
+    int64_t u_off, v_off, tmp_off, p_off, r0, c0;
     int m, n, rank, twin;
 } tlr_tile;
 
 /* One contribution to a block row: kind 0 = F T with F (rows x cols, ld) at
  * f_off of the factor buffer and T (cols x nrhs, ld cols) at src_off*nrhs of
  * the coefficients; kind 1 = D X[src_off..] with D (rows x cols, ld);
- * kind 2 = D^T X[src_off..] with D (cols x rows, ld). */
+ * kind 2 = D^T X[src_off..] with D (cols x rows, ld); kind 3 = the rows
+ * (rows x nrhs, ld) at src_off*nrhs of the coefficient buffer. */
 typedef struct
 {
     int64_t f_off, src_off;
@@ -40,7 +42,8 @@ typedef struct
 } tlr_chunk;
 
 /* Y = alpha * M * X + beta * Y with the plan in device memory (d_tmp holds
- * the coefficients, sum over tiles of 2 r nrhs doubles). */
+ * the coefficients: (sum over tiles of 2 r, plus n per twin tile) x nrhs
+ * doubles). */
 /* Static device scratch of the allocation-free product: its size in
  * doubles, the devices it is set up for, the per-device set-up (called by
  * starsh_hip_init with the device current) and the lookup of its device

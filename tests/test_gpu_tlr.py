@@ -54,6 +54,14 @@ def test_dmml_symmetric_with_stored_near_tiles():
     want1 = O.dmml(st, sz, "S", far, Us, Vs, near, Ds, A[:, :1], 1.0, 0.0,
                    np.zeros((n, 1)))
     assert np.linalg.norm(got1 - want1) <= 1e-13 * np.linalg.norm(want1)
+    # six right-hand sides: a group of four, then a group of two (the
+    # twins' partial rows of the second group at their own offsets)
+    A6 = rng.standard_normal((n, 6))
+    got6 = p.dmml(A6)
+    want6 = O.dmml(st, sz, "S", far, Us, Vs, near, Ds, A6, 1.0, 0.0,
+                   np.zeros((n, 6)))
+    assert np.linalg.norm(got6 - want6) <= 1e-13 * np.linalg.norm(want6)
+    assert max(p.ranks()) > 16      # several 16-column blocks per factor
     p.free_matrix()
 
 
