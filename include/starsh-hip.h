@@ -184,10 +184,11 @@ int starsh_hip_fp64_probe(double *mfma_tflops, double *valu_tflops);
 
 /* Tuning options of the HIP drivers (no reference counterpart):
  *   "workspace_gb"        device workspace budget per batch (default 96);
- *   "mixed_first_bucket"  first rank bucket of the mixed form, an index into
- *                         {32, 64, 128, 256, 512}, or -1 (default) for the
- *                         autotuner's schedule table (1, width 64,
- *                         without an entry);
+ *   "mixed_first_bucket"  first rank bucket of the mixed form, an index
+ *                         0..4 into the ladder's widths ({32, 64, 128, 256,
+ *                         512}; on ladder 1 index 4 is 384), or -1
+ *                         (default) for the autotuner's schedule table (1,
+ *                         width 64, without an entry);
  *   "gemm_bn"             column block of the generated-tile GEMMs: 0 (by
  *                         size, default), 64, 128 or 256;
  *   "gemm_rw"             16-row groups per wave of those GEMMs: 0 (by

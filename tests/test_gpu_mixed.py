@@ -174,3 +174,49 @@ def test_mixed_sigma_noise_and_matern_variants(kw):
     assert np.max(np.abs(rm - rp)) <= 3
     pp.free_matrix()
     pm.free_matrix()
+
+
+def test_mixed_ladder1_3d_matern(tmp_path):
+    """Bucket ladder 1 (a 384-wide bucket; an unfinished tile moves to the
+    width its computed rank needs when its sketch was not saturated) on 3D
+    Matern nu=1.5 from first width 256: tiles that need more than 256
+    columns finish at width 384, with the accuracy of the plain form and
+    its ranks to within 3."""
+    _say("case: mixed ladder 1, 3D Matern nu=1.5 N=32768 nb=2048 tol 1e-7")
+    L = S.lib()
+    trace = tmp_path / "rounds.txt"
+    kw = dict(seed=2, place=S.STARSH_PARTICLES_RAND, kernel="matern",
+              beta=0.1, nu=1.5)
+    fn = S.init("HIP", oversample=16)
+    pp = S.TLRProblem("spatial", 32768, 3, 2048, symm="S", **kw)
+    assert pp.approximate(512, 1e-7, 0, fn) == 0
+    rp = pp.ranks()
+    gp, _, _ = pp.dfe_tiles()
+    assert L.starsh_hip_set_option(b"mixed_first_bucket", C.c_double(3)) == 0
+    assert L.starsh_hip_set_option(b"mixed_ladder", C.c_double(1)) == 0
+    os.environ["STARSH_HIP_MIXED_TRACE"] = str(trace)
+    try:
+        fn = S.init("HIP_MIXED", oversample=16)
+        pm = S.TLRProblem("spatial", 32768, 3, 2048, symm="S", **kw)
+        assert pm.approximate(512, 1e-7, 0, fn) == 0
+    finally:
+        del os.environ["STARSH_HIP_MIXED_TRACE"]
+        L.starsh_hip_set_option(b"mixed_first_bucket", C.c_double(-1))
+        L.starsh_hip_set_option(b"mixed_ladder", C.c_double(-1))
+    rm = pm.ranks()
+    gm, terr, tnorm = pm.dfe_tiles()
+    # round, problem, tile, width, rank, accepted, ...
+    rows = [line.split() for line in trace.read_text().splitlines()]
+    w384 = [r for r in rows if int(r[3]) == 384]
+    _say(f"  plain err {gp:.3e} mixed err {gm:.3e}; {len(rows)} tile "
+         f"rounds, {len(w384)} at width 384; ranks plain "
+         f"{rp.min()}..{rp.max()} mixed {rm.min()}..{rm.max()}")
+    assert len(w384) > 0 and all(int(r[5]) == 1 for r in w384)
+    assert all(int(r[0]) == 0 for r in rows if int(r[3]) == 256)
+    assert pm.tiles() == pp.tiles()
+    assert gm <= 1e-7
+    big = tnorm > 1e-12 * tnorm.max()
+    assert np.all(terr[big] <= 2e-7)
+    assert np.max(np.abs(rm - rp)) <= 3
+    pp.free_matrix()
+    pm.free_matrix()
