@@ -292,3 +292,21 @@ def test_jacobi_sweep_option_is_checked():
             S.STARSH_WRONG_PARAMETER
     assert L.starsh_hip_set_option(b"jacobi_max_sweeps", C.c_double(5)) == 0
     assert L.starsh_hip_set_option(b"jacobi_max_sweeps", C.c_double(0)) == 0
+
+
+def test_mixed_schedule_options_are_checked():
+    """The mixed form's schedule options take their documented values and
+    refuse others, without touching a GPU: mixed_ladder -1 (the autotuner's
+    table), 0 or 1; mixed_first_bucket -1 or an index 0..4;
+    tile_schedule 0 or 1."""
+    L = S.lib()
+    W = S.STARSH_WRONG_PARAMETER
+    for name, good, bad in ((b"mixed_ladder", (0, 1, -1), (2, -2, 5)),
+                            (b"mixed_first_bucket", (0, 4, -1), (5, -2)),
+                            (b"tile_schedule", (0, 1), (2, -1))):
+        for v in bad:
+            assert L.starsh_hip_set_option(name, C.c_double(v)) == W, \
+                (name, v)
+        for v in good:
+            assert L.starsh_hip_set_option(name, C.c_double(v)) == 0, \
+                (name, v)
