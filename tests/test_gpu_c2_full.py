@@ -14,6 +14,8 @@ Plain form (promise: the reference's ranks and errors):
     oracle's error for the same tile.
 Mixed form: global error <= eps, per-tile errors <= 2 eps, ranks within 3
 of the plain form's on the same tiles.
+Product with the mixed form's factors (one and three right-hand sides):
+within 1e-13 relative of the oracle's dmml on the same factors.
 """
 import numpy as np
 import pytest
@@ -115,4 +117,31 @@ def test_c2_mixed_errors_and_ranks(plain):
     assert len(p.near_tiles()) == 0
     for t, r in zip(p.tiles(), ranks.tolist()):
         assert abs(r - plain_rank[t]) <= 3, (t, r, plain_rank[t])
+    p.free_matrix()
+
+
+def test_c2_product_matches_oracle():
+    """The product with the compressed matrix at the headline shape (the
+    mixed form's factors, ranks 7-31, 780 tiles with a transposed twin),
+    one and three right-hand sides, against the oracle's restatement of
+    dmml.c on the same factors."""
+    _say("case: C2 full size, product with the mixed form's factors")
+    fn = S.init("HIP_MIXED", oversample=OS)
+    p = _c2()
+    assert p.approximate(MAXRANK, TOL, 0, fn) == 0
+    st, sz = p.cluster_ranges()
+    far = p.tiles()
+    UV = [p.factors(b) for b in range(len(far))]
+    near = p.near_tiles()
+    Ds = [p.near_block(b) for b in range(len(near))]
+    rng = np.random.default_rng(11)
+    for nrhs in (1, 3):
+        A = rng.standard_normal((64000, nrhs))
+        got = p.dmml(A)
+        want = O.dmml(st, sz, "S", far, [u for u, _ in UV],
+                      [v for _, v in UV], near, Ds, A, 1.0, 0.0,
+                      np.zeros((64000, nrhs)))
+        rel = np.linalg.norm(got - want) / np.linalg.norm(want)
+        _say(f"  nrhs {nrhs}: rel diff {rel:.2e}")
+        assert rel <= 1e-13
     p.free_matrix()
