@@ -117,8 +117,10 @@ __device__ __forceinline__ void factor_pass(const double *F, int len, int r,
     }
 }
 
+// one right-hand side: at most 128 VGPRs, four workgroups per CU, so that
+// the 820 tiles of a 40 x 40 symmetric format run in one round
 template <int NQ>
-__global__ __launch_bounds__(256) void tlr_coef_kernel(
+__global__ __launch_bounds__(256, NQ == 1 ? 4 : 1) void tlr_coef_kernel(
         const tlr_tile *tiles, const double *fac, const double *x, int ldx,
         int nrhs, double *tmp)
 {
