@@ -127,9 +127,11 @@ __global__ __launch_bounds__(256) void gemm_kernel(const TileDesc *tiles,
         for(int b = 0; b < 2; b++)
             acc[a][b] = double4_t{0.0, 0.0, 0.0, 0.0};
 
-    for(int k0 = kbeg; k0 < o.K; k0 += BK)
-    {
-        // ---- stage A (BM x BK) as As[k][i]
+    // operands of K step k0 loaded into registers (ra, rb) one step ahead
+    // of the MFMAs that use them, so the loads of the next step overlap the
+    // products of this one
+    double ra[4], rb[4];
+    auto load_ab = [&](int k0) {
         if(AL == 0)
         {
             const int i = tid & 63, gi = row0+i;
@@ -137,10 +139,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(const TileDesc *tiles,
             for(int s = 0; s < 4; s++)
             {
                 const int kk = (tid >> 6)+4*s, gk = k0+kk;
-                double v = 0.0;
-                if(gi < o.M && gk < o.K)
-                    v = o.A[(int64_t)gk*o.lda+gi];
-                As[kk][i] = v;
+                ra[s] = gi < o.M && gk < o.K ?
+                    o.A[(int64_t)gk*o.lda+gi] : 0.0;
             }
         }
         else
@@ -150,26 +150,39 @@ __global__ __launch_bounds__(256) void gemm_kernel(const TileDesc *tiles,
             for(int s = 0; s < 4; s++)
             {
                 const int i = (tid >> 4)+16*s, gi = row0+i;
-                double v = 0.0;
-                if(gi < o.M && gk < o.K)
-                    v = o.A[(int64_t)gi*o.lda+gk];
-                As[kk][i] = v;
+                ra[s] = gi < o.M && gk < o.K ?
+                    o.A[(int64_t)gi*o.lda+gk] : 0.0;
             }
         }
-        // ---- stage B (BK x BN) as Bs[k][j]; B is column-major K x N
-        {
-            const int kk = tid & 15, gk = k0+kk;
+        // B is column-major K x N
+        const int kk = tid & 15, gk = k0+kk;
 #pragma unroll
-            for(int s = 0; s < 4; s++)
-            {
-                const int j = (tid >> 4)+16*s, gj = col0+j;
-                double v = 0.0;
-                if(gj < o.N && gk < o.K)
-                    v = o.B[(int64_t)gj*o.ldb+gk];
-                Bs[kk][j] = v;
-            }
+        for(int s = 0; s < 4; s++)
+        {
+            const int j = (tid >> 4)+16*s, gj = col0+j;
+            rb[s] = gj < o.N && gk < o.K ? o.B[(int64_t)gj*o.ldb+gk] : 0.0;
         }
+    };
+    // A (BM x BK) as As[k][i], B (BK x BN) as Bs[k][j]
+    auto store_ab = [&]() {
+#pragma unroll
+        for(int s = 0; s < 4; s++)
+        {
+            if(AL == 0)
+                As[(tid >> 6)+4*s][tid & 63] = ra[s];
+            else
+                As[tid & 15][(tid >> 4)+16*s] = ra[s];
+            Bs[tid & 15][(tid >> 4)+16*s] = rb[s];
+        }
+    };
+    if(kbeg < o.K)
+        load_ab(kbeg);
+    for(int k0 = kbeg; k0 < o.K; k0 += BK)
+    {
+        store_ab();
         __syncthreads();
+        if(k0+BK < o.K)
+            load_ab(k0+BK);
 #pragma unroll
         for(int k4 = 0; k4 < BK; k4 += 4)
         {
