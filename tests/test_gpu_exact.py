@@ -132,6 +132,22 @@ def test_rrqr_pivoted_tiles(backend):
     assert st.tile_rounds > st.ntiles
 
 
+@pytest.mark.parametrize("engine,kind,kw", [
+    ("rrqr", "coulomb", {}),
+    ("sdd", "sin", dict(wave_k=50.0, diag=1.0)),
+])
+def test_exact_engines_particles_ragged(engine, kind, kw):
+    """The exact engines on the particle kernels (1/r, sin(kr)/r) of 12
+    spheres x 100 points with tile 256: a ragged last tile row and column
+    (176), self terms at r = 0; ranks and errors against the oracle's
+    dsdd / dqp3 tile loops."""
+    pts = S.spheres(12, 100, 0.02, 3)
+    _say(f"case: {engine} engine, {kind} on 12 spheres x 100 points, "
+         "tile 256 (ragged 176), maxrank 60 os 10 tol 1e-6")
+    _vs_oracle(engine, "HIP", kind, pts.shape[1], 3, 256, 60, 10, 1e-6,
+               points=pts, **kw)
+
+
 def test_sdd_matern3d_131072_sampled():
     """The SVD engine at the 3D Matern workload's full size (N=131072, tile
     2048, maxrank 512, oversample 16, eps 1e-7): error bound over all tiles,
