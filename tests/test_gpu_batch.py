@@ -31,12 +31,23 @@ def _free(ps):
         p.free_matrix()
 
 
-@pytest.mark.parametrize("backend", ["HIP", "HIP_MIXED"])
-def test_batch_of_identical_problems_equals_single_call(backend):
+@pytest.mark.parametrize("backend,engine", [
+    ("HIP", "RSVD"), ("HIP_MIXED", "RSVD"), ("HIP", "SVD"), ("HIP", "RRQR"),
+])
+def test_batch_of_identical_problems_equals_single_call(backend, engine):
     """Two copies of one problem in a batch give exactly the single-call
-    ranks and factors (same tile shapes, so the same launches per tile)."""
-    _say(f"case: {backend} batch of 2 identical 2D exp N=3000 nb=300")
-    fn = S.init(backend, oversample=10)
+    ranks and factors (same tile shapes, so the same launches per tile),
+    for the randomized engine and the exact ones."""
+    _say(f"case: {backend} {engine} batch of 2 identical 2D exp N=3000 "
+         "nb=300")
+    try:
+        _identical_pair(backend, engine)
+    finally:
+        S.init("HIP", lrengine="RSVD")
+
+
+def _identical_pair(backend, engine):
+    fn = S.init(backend, oversample=10, lrengine=engine)
     spec = ("spatial", "exp", 3000, 2, 300, 0.1, 7)
     single = _problems([spec])[0]
     assert single.approximate(150, 1e-7, 0, fn) == 0
