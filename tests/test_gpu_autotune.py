@@ -277,3 +277,21 @@ def test_schedule_file_round_trip(tmp_path):
     assert rep["early_dense"] == len(marked)
     assert rep["tile_rounds"] == nfar - len(marked)
     assert far_rep == {k: v for k, v in far_b4.items() if k not in marked}
+
+
+def test_schedule_file_malformed_is_ignored(tmp_path):
+    """A schedule file the loader cannot read (an entry in an older
+    22-field layout, then text) is ignored: a new process compresses as it
+    does without a file (same ranks, same rounds, error within the
+    bound)."""
+    _say("case: schedule file, absent")
+    none = _child("replay", tmp_path / "absent.txt")
+    bad = tmp_path / "bad.txt"
+    bad.write_text("0 2 0.2589 0.5 0 1 1 1 20000 1000 120 16 1e-07 1 1 4 "
+                   "0 0 0 0 12345 0 -\nnot a schedule\n")
+    _say("case: schedule file, malformed")
+    r = _child("replay", bad)
+    assert r["replay"]["err"] <= 1e-7
+    assert r["replay"]["far"] == none["replay"]["far"]
+    assert r["replay"]["tile_rounds"] == none["replay"]["tile_rounds"]
+    assert r["replay"]["early_dense"] == none["replay"]["early_dense"]
