@@ -205,6 +205,10 @@ int starsh_hip_fp64_probe(double *mfma_tflops, double *valu_tflops);
  *                         converged within it is a false far-field tile
  *                         (rank -1), as for a failed dgesdd in
  *                         src/backends/sequential/dense/drsdd.c:78;
+ *   "mixed_trunc"         truncation factor eta of the mixed form in (0,
+ *                         0.5] (default 1e-5): rows of B whose tail is at
+ *                         most (eta tol)^2 |B|_F^2 are dropped before the
+ *                         core SVD and counted in the rank rule's tail;
  *   "mixed_early_dense"   the mixed form declares a tile false far-field
  *                         when its rank, extrapolated from the decay of
  *                         its sketch's singular values, exceeds this
